@@ -1,0 +1,173 @@
+"""Pressure-solve benchmark: solves/s to 1e-10 relative residual at 256^3.
+
+One step = one pressure solve of the 256^3 channel workload (x/z periodic,
+y walls, tanh-stretched y, stretch 2; seeded spectral RHS) from a zero initial
+guess, with opr, rhs and sol already resident in HBM (device entry points of
+libngapsolve.so). The headline mode is the FP64 flexible PCG with the FP32
+V-cycle; the plain FP64 method is timed beside it. Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
+                  [--no-cpu] [--cpu-threads T] [--roofline-reps R]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "pressure solves/sec to 1e-10 rel. residual at 256^3; HBM GB/s vs roofline"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--workload", default="channel_spectral")
+    ap.add_argument("--rcvg", type=float, default=1e-10)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--roofline-reps", type=int, default=20)
+    ap.add_argument("--skip-fp64", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    ws = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or ws > 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus > 1 is not supported (the solve would "
+                                   "shard as a 3-D block decomposition with halo exchange, not implemented here)"}))
+        return 2
+    import torch
+
+    from nga2_amd import _lib, inputs
+    from nga2_amd._lib import FP64, MIXED
+    from nga2_amd.linsol import DeviceSolve
+
+    assert torch.cuda.is_available(), "bench needs a GPU"
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    P = inputs.WORKLOADS[args.workload](n=args.n)
+    cfg = P.cfg
+    log(f"[bench] generated {P.name} in {time.time() - t0:.1f}s")
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
+    d_sol = torch.zeros_like(d_rhs)
+    N = cfg.nx * cfg.ny * cfg.nz
+
+    def run_mode(prec, steps, warmup):
+        ds = DeviceSolve(cfg)
+        ts = time.perf_counter()
+        ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=prec)
+        torch.cuda.synchronize()
+        setup_s = time.perf_counter() - ts
+        for _ in range(warmup):
+            d_sol.zero_()
+            ds.solve(d_rhs, d_sol)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            d_sol.zero_()
+            it, _ = ds.solve(d_rhs, d_sol)
+            its.append(it)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        st = ds.stats()
+        sol = d_sol.cpu().numpy().reshape(cfg.shape, order="F")
+        return ds, dict(elapsed=el, setup_s=setup_s, its=its, stats=st, sol=sol)
+
+    log("[bench] mixed-precision solves")
+    ds_mixed, mixed = run_mode(MIXED, args.steps, args.warmup)
+    log(f"[bench] mixed: {args.steps / mixed['elapsed']:.2f} solves/s, it={mixed['its'][0]}")
+    fp64 = None
+    if not args.skip_fp64:
+        log("[bench] plain FP64 solves")
+        ds_fp64, fp64 = run_mode(FP64, max(1, args.steps // 2), 1)
+        log(f"[bench] fp64: {len(fp64['its']) / fp64['elapsed']:.2f} solves/s, it={fp64['its'][0]}")
+
+    # roofline of the dominant kernel (fine-level FP32 red/black GS colour pass)
+    hk = ds_mixed.hooks()
+    rb_ms, rb_bytes = hk.time_kernel(_lib.K_RBGS, prec=1, reps=args.roofline_reps)
+    sp_ms, sp_bytes = hk.time_kernel(_lib.K_SPMV, prec=0, reps=args.roofline_reps)
+    rb_gbs = rb_bytes / (rb_ms * 1e-3) / 1e9
+    sp_gbs = sp_bytes / (sp_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(rb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(rb_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_rbgs<float> fine-level colour pass", "avg_ms": rb_ms,
+                "algorithmic_bytes": rb_bytes,
+                "spmv_fp64": {"achieved": round(sp_gbs, 1), "frac": round(sp_gbs / HBM_PEAK_GBS, 4), "avg_ms": sp_ms}}
+
+    # CPU baseline + FP64 oracle error
+    cpu = None
+    err_mixed = err_fp64 = None
+    if not args.no_cpu:
+        from oracle.orc import Oracle, interior, num_threads
+
+        if args.cpu_threads > 0:
+            os.environ["OMP_NUM_THREADS"] = str(args.cpu_threads)
+        log("[bench] CPU oracle (FP64 PCG-PFMG, OpenMP)")
+        ts = time.perf_counter()
+        o = Oracle.from_cfg(cfg, P.opr)
+        osetup = time.perf_counter() - ts
+        b = interior(P.rhs, cfg)
+        ts = time.perf_counter()
+        xr, st_r = o.solve(b, rcvg=args.rcvg, maxit=500)
+        osolve = time.perf_counter() - ts
+        cpu = {"value": round(1.0 / osolve, 4), "unit": "solves/s", "cores": num_threads(), "kind": "port",
+               "sample": f"one full solve of {P.name} to {args.rcvg:g} (setup {osetup:.1f}s excluded)",
+               "iters": st_r["it"], "solve_s": round(osolve, 3)}
+        xr -= xr.mean()
+
+        def err(sol):
+            x = interior(sol, cfg)
+            x = x - x.mean()
+            return float(np.linalg.norm(x - xr) / np.linalg.norm(xr))
+
+        err_mixed = err(mixed["sol"])
+        if fp64:
+            err_fp64 = err(fp64["sol"])
+        o.close()
+
+    steps = args.steps
+    el = mixed["elapsed"]
+    value = steps / el
+    out = {
+        "metric": METRIC, "value": round(value, 4), "unit": "solves/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * el / steps, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
+        "config": {"workload": f"{P.name}: {cfg.nx}x{cfg.ny}x{cfg.nz}, x/z periodic, y walls tanh stretch 2, "
+                               f"spectral RHS seed 2002; FP64 flexible PCG + FP32 PFMG-style V(1,1), "
+                               f"rcvg {args.rcvg:g}, zero guess, inputs resident in HBM",
+                   "cells": N, "precision": "mixed"},
+        "iters": mixed["its"][0], "true_rerr": mixed["stats"]["true_rerr"], "err_vs_fp64_oracle": err_mixed,
+        "setup_ms": round(1e3 * mixed["setup_s"], 2), "levels": mixed["stats"]["nlevels"],
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    if fp64:
+        out["fp64_plain"] = {"value": round(len(fp64["its"]) / fp64["elapsed"], 4), "unit": "solves/s",
+                             "iters": fp64["its"][0], "true_rerr": fp64["stats"]["true_rerr"],
+                             "err_vs_fp64_oracle": err_fp64,
+                             "ms_per_solve": round(1e3 * fp64["elapsed"] / len(fp64["its"]), 3)}
+        out["speedup_mixed_vs_fp64"] = round(out["value"] / out["fp64_plain"]["value"], 3)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,1494 @@
+// C ABI, setup (hierarchy build) and the PCG driver of the MI355X pressure
+// Poisson solver. Replaces hypre_str with method pcg_pfmg
+// (src/solver/hypre_str_class.f90:264-284 in the NGA2 tree).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
+#include <vector>
+
+#include "ngap_internal.h"
+#include "ngapsolve.h"
+#include "ngapsolve_test.h"
+
+using namespace ngap;
+
+namespace {
+
+// ---------------------------------------------------------------- scalar slots
+enum { S_RR = 0, S_BB, S_RHO, S_ALPHA, S_BETA, S_PQ, S_RHO0, S_BAD, S_RRT, S_ASYM, S_STR0, S_STR1, S_STR2, S_NSCAL };
+constexpr int NSCAL_ALLOC = 16;
+
+struct Ghost {
+    long long nxo, nyo;
+    int oi, oj, ok;
+};
+
+struct Slots {
+    int s[7];  // stencil slot of centre, +x, -x, +y, -y, +z, -z
+    int nst;
+};
+
+__device__ __forceinline__ long long gidx(const Ghost &h, int i, int j, int k) {
+    return (long long)(i + h.oi) + h.nxo * ((long long)(j + h.oj) + h.nyo * (long long)(k + h.ok));
+}
+
+__device__ __forceinline__ void decomp0(const Geo &g, long long p, int &i, int &j, int &k) {
+    i = (int)(p % g.n0);
+    long long r = p / g.n0;
+    j = (int)(r % g.n1);
+    k = (int)(r / g.n1);
+}
+
+__device__ __forceinline__ long long lin0(const Geo &g, int i, int j, int k) {
+    return (long long)i + (long long)g.n0 * ((long long)j + (long long)g.n1 * k);
+}
+
+// ---------------------------------------------------------------- setup kernels
+
+// Level-0 symmetric storage from the ghosted AoS opr (hypre_str_class.f90:
+// 171-179: rows with opr(1)==0 become identity rows).
+__global__ void __launch_bounds__(NT) k_level0(Geo g, Ghost gh, Slots sl, const double *__restrict__ opr,
+                                               double *__restrict__ c, double *__restrict__ a0,
+                                               double *__restrict__ a1, double *__restrict__ a2,
+                                               unsigned char *__restrict__ part, double *__restrict__ p_asym,
+                                               double *__restrict__ p_bad) {
+    __shared__ double sm[NT / 64];
+    double asym = 0.0, bad = 0.0;
+    double *a[3] = {a0, a1, a2};
+    const int n3[3] = {g.n0, g.n1, g.n2}, per3[3] = {g.p0, g.p1, g.p2};
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
+        int c3[3];
+        decomp0(g, p, c3[0], c3[1], c3[2]);
+        const double *A = opr + gidx(gh, c3[0], c3[1], c3[2]) * sl.nst;
+        double v[7];
+        for (int s = 0; s < 7; s++) {
+            v[s] = A[sl.s[s]];
+            if (!isfinite(v[s])) bad += 1.0;
+        }
+        const bool pt = fabs(v[0]) > 0.0;
+        part[p] = pt ? 1 : 0;
+        double cc = pt ? v[0] : 1.0;
+        for (int d = 0; d < 3; d++) {
+            double am = v[2 + 2 * d], ap = v[1 + 2 * d];
+            if (n3[d] == 1) {
+                if (per3[d] && pt) cc += am + ap;
+                a[d][p] = 0.0;
+                continue;
+            }
+            int q = nbm(c3[d], n3[d], per3[d]);
+            if (q < 0 || !pt) {
+                a[d][p] = 0.0;
+                continue;
+            }
+            int e3[3] = {c3[0], c3[1], c3[2]};
+            e3[d] = q;
+            const double *B = opr + gidx(gh, e3[0], e3[1], e3[2]) * sl.nst;
+            if (!(fabs(B[sl.s[0]]) > 0.0)) {
+                a[d][p] = 0.0;
+                continue;
+            }
+            double apn = B[sl.s[1 + 2 * d]];
+            a[d][p] = 0.5 * (am + apn);
+            double rel = fabs(am - apn) / fabs(v[0]);
+            if (rel > asym) asym = rel;
+        }
+        c[p] = cc;
+    }
+    double m = block_max(asym, sm);
+    double b = block_sum(bad, sm);
+    if (threadIdx.x == 0) {
+        p_asym[blockIdx.x] = m;
+        p_bad[blockIdx.x] = b;
+    }
+}
+
+// ---------------------------------------------------------------- solve kernels
+
+// dst = part ? src(ghosted) : 0; counts non-finite values of participating cells
+__global__ void __launch_bounds__(NT) k_gather(Geo g, Ghost gh, const double *__restrict__ src,
+                                               const unsigned char *__restrict__ part, double *__restrict__ dst,
+                                               double *__restrict__ p_bad) {
+    __shared__ double sm[NT / 64];
+    double bad = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
+        int i, j, k;
+        decomp0(g, p, i, j, k);
+        double v = part[p] ? src[gidx(gh, i, j, k)] : 0.0;
+        if (!isfinite(v)) bad += 1.0;
+        dst[p] = v;
+    }
+    if (p_bad) {
+        double b = block_sum(bad, sm);
+        if (threadIdx.x == 0) p_bad[blockIdx.x] = b;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_scatter(Geo g, Ghost gh, const double *__restrict__ src,
+                                                double *__restrict__ dst) {
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
+        int i, j, k;
+        decomp0(g, p, i, j, k);
+        dst[gidx(gh, i, j, k)] = src[p];
+    }
+}
+
+// r = b - A x with the exact 7-array operator (identity rows where the
+// centre is 0; couplings out of a non-periodic box ignored); partial ||r||^2.
+__global__ void __launch_bounds__(NT) k_exact_resid(Geo g, Ghost gh, Slots sl, const double *__restrict__ opr,
+                                                    const double *__restrict__ b, const double *__restrict__ x,
+                                                    double *__restrict__ r, double *__restrict__ part) {
+    __shared__ double sm[NT / 64];
+    double acc = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
+        int i, j, k;
+        decomp0(g, p, i, j, k);
+        const double *A = opr + gidx(gh, i, j, k) * sl.nst;
+        double s;
+        double a0 = A[sl.s[0]];
+        if (!(fabs(a0) > 0.0)) {
+            s = x[p];
+        } else {
+            s = a0 * x[p];
+            int ip = nbp(i, g.n0, g.p0), im = nbm(i, g.n0, g.p0);
+            int jp = nbp(j, g.n1, g.p1), jm = nbm(j, g.n1, g.p1);
+            int kp = nbp(k, g.n2, g.p2), km = nbm(k, g.n2, g.p2);
+            if (ip >= 0) s += A[sl.s[1]] * x[lin0(g, ip, j, k)];
+            if (im >= 0) s += A[sl.s[2]] * x[lin0(g, im, j, k)];
+            if (jp >= 0) s += A[sl.s[3]] * x[lin0(g, i, jp, k)];
+            if (jm >= 0) s += A[sl.s[4]] * x[lin0(g, i, jm, k)];
+            if (kp >= 0) s += A[sl.s[5]] * x[lin0(g, i, j, kp)];
+            if (km >= 0) s += A[sl.s[6]] * x[lin0(g, i, j, km)];
+        }
+        double rv = b[p] - s;
+        r[p] = rv;
+        acc += rv * rv;
+    }
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(NT) k_sumsq(long long n, const double *__restrict__ x, double *__restrict__ part) {
+    __shared__ double sm[NT / 64];
+    double acc = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT) acc += x[p] * x[p];
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename Tx, typename Ty>
+__global__ void __launch_bounds__(NT) k_dot(long long n, const Tx *__restrict__ x, const Ty *__restrict__ y,
+                                            double *__restrict__ part) {
+    __shared__ double sm[NT / 64];
+    double acc = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT)
+        acc += (double)x[p] * (double)y[p];
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2 ; optional FP32 copy of r
+__global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restrict__ x, double *__restrict__ r,
+                                                  const double *__restrict__ p, const double *__restrict__ q,
+                                                  const double *__restrict__ scal, float *__restrict__ r32,
+                                                  double *__restrict__ part) {
+    __shared__ double sm[NT / 64];
+    const double a = scal[S_ALPHA];
+    double acc = 0.0;
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+        double rv = r[i];
+        if (a != 0.0) {
+            x[i] += a * p[i];
+            rv -= a * q[i];
+            r[i] = rv;
+        }
+        acc += rv * rv;
+        if (r32) r32[i] = (float)rv;
+    }
+    double t = block_sum(acc, sm);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// <r,z> and (optionally) <z,q>
+template <typename Tz>
+__global__ void __launch_bounds__(NT) k_dot_rz(long long n, const double *__restrict__ r, const Tz *__restrict__ z,
+                                               const double *__restrict__ q, double *__restrict__ p_rz,
+                                               double *__restrict__ p_zq) {
+    __shared__ double sm[NT / 64];
+    double a = 0.0, b = 0.0;
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+        double zv = (double)z[i];
+        a += r[i] * zv;
+        if (q) b += zv * q[i];
+    }
+    double t = block_sum(a, sm);
+    if (threadIdx.x == 0) p_rz[blockIdx.x] = t;
+    if (q) {
+        double u = block_sum(b, sm);
+        if (threadIdx.x == 0) p_zq[blockIdx.x] = u;
+    }
+}
+
+template <typename Tz>
+__global__ void __launch_bounds__(NT) k_update_p(long long n, double *__restrict__ p, const Tz *__restrict__ z,
+                                                 const double *__restrict__ scal, int copy) {
+    const double be = copy ? 0.0 : scal[S_BETA];
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+        double zv = (double)z[i];
+        p[i] = copy ? zv : zv + be * p[i];
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_copy(long long n, const double *__restrict__ s, double *__restrict__ d) {
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(NT) k_zero64(long long n, double *__restrict__ d) {
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) d[i] = 0.0;
+}
+
+__device__ __forceinline__ double sum_parts(const double *__restrict__ parts, int n, double *sm) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += NT) s += parts[q];
+    return block_sum(s, sm);
+}
+
+__device__ __forceinline__ double max_parts(const double *__restrict__ parts, int n, double *sm) {
+    double s = 0.0;
+    for (int q = threadIdx.x; q < n; q += NT) s = fmax(s, parts[q]);
+    return block_max(s, sm);
+}
+
+__global__ void __launch_bounds__(NT) k_fin_sum(const double *__restrict__ parts, int n, double *__restrict__ scal,
+                                                int slot) {
+    __shared__ double sm[NT / 64];
+    double s = sum_parts(parts, n, sm);
+    if (threadIdx.x == 0) scal[slot] = s;
+}
+
+__global__ void __launch_bounds__(NT) k_fin_max(const double *__restrict__ parts, int n, double *__restrict__ scal,
+                                                int slot) {
+    __shared__ double sm[NT / 64];
+    double s = max_parts(parts, n, sm);
+    if (threadIdx.x == 0) scal[slot] = s;
+}
+
+// alpha = rho / <p,Ap>; 0 and a flag on breakdown
+__global__ void __launch_bounds__(NT) k_fin_alpha(const double *__restrict__ parts, int n, double *__restrict__ scal,
+                                                  int *__restrict__ flags) {
+    __shared__ double sm[NT / 64];
+    double pq = sum_parts(parts, n, sm);
+    if (threadIdx.x == 0) {
+        double rho = scal[S_RHO];
+        double a = rho / pq;
+        if (!(pq > 0.0) || !(rho > 0.0) || !isfinite(a)) {
+            a = 0.0;
+            flags[0] |= 1;
+        }
+        scal[S_PQ] = pq;
+        scal[S_ALPHA] = a;
+    }
+}
+
+// rho = <r,z> after a restart
+__global__ void __launch_bounds__(NT) k_fin_rho(const double *__restrict__ parts, int n, double *__restrict__ scal,
+                                                int *__restrict__ flags, int first) {
+    __shared__ double sm[NT / 64];
+    double rz = sum_parts(parts, n, sm);
+    if (threadIdx.x == 0) {
+        if (!(rz > 0.0) || !isfinite(rz)) flags[0] |= 2;
+        scal[S_RHO] = rz;
+        if (first) scal[S_RHO0] = rz;
+    }
+}
+
+// beta (plain: <r+,z+>/<r,z>; flexible: -alpha <z+,q>/<r,z>)
+__global__ void __launch_bounds__(NT) k_fin_beta(const double *__restrict__ p_rz, const double *__restrict__ p_zq,
+                                                 int n, double *__restrict__ scal, int *__restrict__ flags,
+                                                 int flexible) {
+    __shared__ double sm[NT / 64];
+    double rz = sum_parts(p_rz, n, sm);
+    double zq = flexible ? sum_parts(p_zq, n, sm) : 0.0;
+    if (threadIdx.x == 0) {
+        double rho = scal[S_RHO];
+        double be = flexible ? -scal[S_ALPHA] * zq / rho : rz / rho;
+        if (!(rz > 0.0) || !isfinite(be)) {
+            be = 0.0;
+            flags[0] |= 2;
+        }
+        scal[S_BETA] = be;
+        scal[S_RHO] = rz;
+    }
+}
+
+__global__ void k_clear_flags(int *flags) {
+    if (threadIdx.x == 0) flags[0] = 0;
+}
+
+double now_ms() {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+// ====================================================================== handle
+
+struct ngap_t {
+    ngap_grid g{};
+    int nst = 0;
+    std::vector<int> stc;
+    int method = 0;
+    Slots slots{};
+    bool init_done = false, setup_done = false;
+    int maxit = 100;
+    double rcvg = 1e-10;
+    int maxlevel = 0;
+    ngap_opts opts{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Ghost gh{};
+    long long nghost = 0;
+    Geo g0{};
+    std::vector<Level> lev;
+    int nlev = 0;
+    double *opr = nullptr;  // own ghosted copy of the exact operator
+    unsigned char *part = nullptr;
+    double *rhs_g = nullptr, *sol_g = nullptr;  // ghosted staging for host variants
+    double *b = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+    float *r32 = nullptr, *z32 = nullptr;
+    double *parts = nullptr;  // 4 * MAX_PARTS
+    double *scal = nullptr;
+    int *flags = nullptr;
+    double *h_scal = nullptr;  // pinned
+    int *h_flags = nullptr;    // pinned
+    double *Ainv = nullptr;
+    int nc = 0;
+    int coarse_dense = 0, coarse_singular = 0;
+    ngap_stats st{};
+    long long dev_bytes = 0;
+    std::vector<void *> base_allocs;
+};
+
+namespace {
+
+hipError_t g_last_hip = hipSuccess;
+
+#define HIPCHK(expr)                                       \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) {                            \
+            g_last_hip = e_;                               \
+            return e_ == hipErrorOutOfMemory ? NGAP_ERR_ALLOC : NGAP_ERR_HIP; \
+        }                                                  \
+    } while (0)
+
+int dmalloc(ngap_t *h, void **ptr, size_t bytes) {
+    *ptr = nullptr;
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(ptr, bytes);
+    if (e != hipSuccess) {
+        g_last_hip = e;
+        *ptr = nullptr;
+        return NGAP_ERR_ALLOC;
+    }
+    h->dev_bytes += (long long)bytes;
+    return 0;
+}
+
+void dfree(ngap_t *h, void *ptr, size_t bytes) {
+    if (ptr) {
+        (void)hipFree(ptr);
+        h->dev_bytes -= (long long)bytes;
+    }
+}
+
+template <typename T>
+int alloc_lvl_data(ngap_t *h, LevelData<T> &D, long long N, bool work) {
+    size_t b = sizeof(T) * (size_t)N;
+    int rc = 0;
+    rc |= dmalloc(h, (void **)&D.c, b);
+    rc |= dmalloc(h, (void **)&D.a0, b);
+    rc |= dmalloc(h, (void **)&D.a1, b);
+    rc |= dmalloc(h, (void **)&D.a2, b);
+    rc |= dmalloc(h, (void **)&D.pm, b);
+    rc |= dmalloc(h, (void **)&D.pp, b);
+    if (work) {
+        rc |= dmalloc(h, (void **)&D.f, b);
+        rc |= dmalloc(h, (void **)&D.v, b);
+    }
+    return rc ? NGAP_ERR_ALLOC : 0;
+}
+
+template <typename T>
+void free_lvl_data(ngap_t *h, LevelData<T> &D, long long N) {
+    size_t b = sizeof(T) * (size_t)N;
+    dfree(h, D.c, b);
+    dfree(h, D.a0, b);
+    dfree(h, D.a1, b);
+    dfree(h, D.a2, b);
+    dfree(h, D.pm, b);
+    dfree(h, D.pp, b);
+    dfree(h, D.f, b);
+    dfree(h, D.v, b);
+    D = LevelData<T>();
+}
+
+void free_level(ngap_t *h, Level &L) {
+    free_lvl_data(h, L.d, L.g.N);
+    free_lvl_data(h, L.s, L.g.N);
+}
+
+void free_all(ngap_t *h) {
+    for (auto &L : h->lev) free_level(h, L);
+    h->lev.clear();
+    h->nlev = 0;
+    if (h->Ainv) {
+        dfree(h, h->Ainv, sizeof(double) * (size_t)h->nc * h->nc);
+        h->Ainv = nullptr;
+    }
+    size_t N = (size_t)h->g0.N;
+    dfree(h, h->opr, sizeof(double) * (size_t)h->nghost * h->nst);
+    dfree(h, h->part, N);
+    dfree(h, h->rhs_g, sizeof(double) * (size_t)h->nghost);
+    dfree(h, h->sol_g, sizeof(double) * (size_t)h->nghost);
+    for (double **v : {&h->b, &h->x, &h->r, &h->z, &h->p, &h->q}) {
+        dfree(h, *v, sizeof(double) * N);
+        *v = nullptr;
+    }
+    dfree(h, h->r32, sizeof(float) * N);
+    dfree(h, h->z32, sizeof(float) * N);
+    dfree(h, h->parts, sizeof(double) * 4 * MAX_PARTS);
+    dfree(h, h->scal, sizeof(double) * NSCAL_ALLOC);
+    dfree(h, h->flags, sizeof(int) * 4);
+    h->opr = nullptr;
+    h->part = nullptr;
+    h->rhs_g = h->sol_g = nullptr;
+    h->r32 = h->z32 = nullptr;
+    h->parts = h->scal = nullptr;
+    h->flags = nullptr;
+}
+
+// fine-level and PCG buffers (depend only on the grid)
+int ensure_base(ngap_t *h) {
+    if (h->opr) return 0;
+    size_t N = (size_t)h->g0.N;
+    int rc = 0;
+    rc |= dmalloc(h, (void **)&h->opr, sizeof(double) * (size_t)h->nghost * h->nst);
+    rc |= dmalloc(h, (void **)&h->part, N);
+    rc |= dmalloc(h, (void **)&h->rhs_g, sizeof(double) * (size_t)h->nghost);
+    rc |= dmalloc(h, (void **)&h->sol_g, sizeof(double) * (size_t)h->nghost);
+    for (double **v : {&h->b, &h->x, &h->r, &h->z, &h->p, &h->q}) rc |= dmalloc(h, (void **)v, sizeof(double) * N);
+    rc |= dmalloc(h, (void **)&h->r32, sizeof(float) * N);
+    rc |= dmalloc(h, (void **)&h->z32, sizeof(float) * N);
+    rc |= dmalloc(h, (void **)&h->parts, sizeof(double) * 4 * MAX_PARTS);
+    rc |= dmalloc(h, (void **)&h->scal, sizeof(double) * NSCAL_ALLOC);
+    rc |= dmalloc(h, (void **)&h->flags, sizeof(int) * 4);
+    if (rc) {
+        free_all(h);
+        return NGAP_ERR_ALLOC;
+    }
+    if (!h->h_scal) {
+        HIPCHK(hipHostMalloc((void **)&h->h_scal, sizeof(double) * NSCAL_ALLOC, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&h->h_flags, sizeof(int) * 4, hipHostMallocDefault));
+    }
+    HIPCHK(hipMemsetAsync(h->scal, 0, sizeof(double) * NSCAL_ALLOC, h->stream));
+    HIPCHK(hipMemsetAsync(h->flags, 0, sizeof(int) * 4, h->stream));
+    return 0;
+}
+
+int read_scal(ngap_t *h) {
+    HIPCHK(hipMemcpyAsync(h->h_scal, h->scal, sizeof(double) * NSCAL_ALLOC, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(h->h_flags, h->flags, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int can_coarsen(const Geo &g, int d) {
+    int n = d == 0 ? g.n0 : (d == 1 ? g.n1 : g.n2);
+    int per = d == 0 ? g.p0 : (d == 1 ? g.p1 : g.p2);
+    if (per) return (n % 2 == 0) && n >= 4;
+    return n >= 2;
+}
+
+// ---------------------------------------------------------------- coarsest (host)
+
+// Dense coarsest operator -> (pseudo-)inverse, FP64, on the host.
+int build_coarsest_inverse(ngap_t *h) {
+    Level &C = h->lev[h->nlev - 1];
+    const Geo &g = C.g;
+    int n = (int)g.N;
+    std::vector<double> c(n), a0(n), a1(n), a2(n);
+    HIPCHK(hipMemcpy(c.data(), C.d.c, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(a0.data(), C.d.a0, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(a1.data(), C.d.a1, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(a2.data(), C.d.a2, sizeof(double) * n, hipMemcpyDeviceToHost));
+    const double *a[3] = {a0.data(), a1.data(), a2.data()};
+    const int n3[3] = {g.n0, g.n1, g.n2}, p3[3] = {g.p0, g.p1, g.p2};
+    std::vector<double> M((size_t)n * n, 0.0);
+    double ssum = 0.0, csum = 0.0;
+    auto lin = [&](int i, int j, int k) { return (long long)i + (long long)g.n0 * ((long long)j + (long long)g.n1 * k); };
+    for (int k = 0; k < g.n2; k++)
+        for (int j = 0; j < g.n1; j++)
+            for (int i = 0; i < g.n0; i++) {
+                long long p = lin(i, j, k);
+                M[p * n + p] += c[p];
+                int c3[3] = {i, j, k};
+                double rs = c[p];
+                for (int d = 0; d < 3; d++) {
+                    int qm = c3[d] > 0 ? c3[d] - 1 : (p3[d] ? n3[d] - 1 : -1);
+                    if (qm >= 0) {
+                        int e3[3] = {i, j, k};
+                        e3[d] = qm;
+                        long long pq = lin(e3[0], e3[1], e3[2]);
+                        M[p * n + pq] += a[d][p];
+                        M[pq * n + p] += a[d][p];
+                    }
+                    rs += a[d][p];
+                    int qp = c3[d] < n3[d] - 1 ? c3[d] + 1 : (p3[d] ? 0 : -1);
+                    if (qp >= 0) {
+                        int e3[3] = {i, j, k};
+                        e3[d] = qp;
+                        rs += a[d][lin(e3[0], e3[1], e3[2])];
+                    }
+                }
+                ssum += std::fabs(rs);
+                csum += std::fabs(c[p]);
+            }
+    h->coarse_singular = ssum <= 1e-10 * csum;
+    if (h->coarse_singular) {
+        double gam = csum / ((double)n * n);
+        for (auto &m : M) m += gam;
+    }
+    // Cholesky (lower)
+    std::vector<double> L((size_t)n * n, 0.0);
+    for (int j = 0; j < n; j++) {
+        double s = M[(size_t)j * n + j];
+        for (int k = 0; k < j; k++) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+        if (!(s > 0.0)) return NGAP_ERR_COARSE;
+        double ljj = std::sqrt(s);
+        L[(size_t)j * n + j] = ljj;
+        for (int i = j + 1; i < n; i++) {
+            double t = M[(size_t)i * n + j];
+            for (int k = 0; k < j; k++) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
+            L[(size_t)i * n + j] = t / ljj;
+        }
+    }
+    // inverse column by column (symmetric result)
+    std::vector<double> X((size_t)n * n, 0.0), y(n);
+    for (int col = 0; col < n; col++) {
+        for (int i = 0; i < n; i++) {
+            double s = (i == col) ? 1.0 : 0.0;
+            for (int k = 0; k < i; k++) s -= L[(size_t)i * n + k] * y[k];
+            y[i] = s / L[(size_t)i * n + i];
+        }
+        for (int i = n - 1; i >= 0; i--) {
+            double s = y[i];
+            for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * y[k];
+            y[i] = s / L[(size_t)i * n + i];
+        }
+        for (int i = 0; i < n; i++) X[(size_t)col * n + i] = y[i];
+    }
+    if (h->Ainv && h->nc != n) {
+        dfree(h, h->Ainv, sizeof(double) * (size_t)h->nc * h->nc);
+        h->Ainv = nullptr;
+    }
+    h->nc = n;
+    if (!h->Ainv && dmalloc(h, (void **)&h->Ainv, sizeof(double) * (size_t)n * n)) return NGAP_ERR_ALLOC;
+    HIPCHK(hipMemcpy(h->Ainv, X.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------- V-cycle
+
+template <typename T>
+LevelData<T> &data(Level &L);
+template <>
+LevelData<double> &data<double>(Level &L) { return L.d; }
+template <>
+LevelData<float> &data<float>(Level &L) { return L.s; }
+
+template <typename T>
+void coarse_solve(ngap_t *h, const T *f, T *v) {
+    Level &C = h->lev[h->nlev - 1];
+    LevelData<T> &D = data<T>(C);
+    if (h->coarse_dense) {
+        launch_dense<T, T>(h->stream, (int)C.g.N, h->Ainv, f, v);
+    } else {
+        for (int s = 0; s < 8; s++) {
+            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 0, s == 0);
+            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
+            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
+            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 0, 0);
+        }
+    }
+}
+
+template <typename T>
+void vcycle(ngap_t *h, const T *f0, T *v0) {
+    const int nl = h->nlev;
+    hipStream_t s = h->stream;
+    for (int l = 0; l < nl - 1; l++) {
+        Level &L = h->lev[l];
+        LevelData<T> &D = data<T>(L);
+        const T *f = l == 0 ? f0 : D.f;
+        T *v = l == 0 ? v0 : D.v;
+        for (int sw = 0; sw < h->opts.nu_pre; sw++) {
+            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 0, sw == 0);
+            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
+        }
+        Level &Cn = h->lev[l + 1];
+        LevelData<T> &DC = data<T>(Cn);
+        launch_resid_restrict<T>(s, L.g, Cn.g, L.cdir, D.c, D.a0, D.a1, D.a2, f, v, DC.pm, DC.pp, DC.f);
+    }
+    {
+        Level &C = h->lev[nl - 1];
+        LevelData<T> &D = data<T>(C);
+        coarse_solve<T>(h, nl == 1 ? f0 : D.f, nl == 1 ? v0 : D.v);
+    }
+    for (int l = nl - 2; l >= 0; l--) {
+        Level &L = h->lev[l];
+        LevelData<T> &D = data<T>(L);
+        const T *f = l == 0 ? f0 : D.f;
+        T *v = l == 0 ? v0 : D.v;
+        Level &Cn = h->lev[l + 1];
+        LevelData<T> &DC = data<T>(Cn);
+        launch_prolong<T>(s, L.g, Cn.g, L.cdir, DC.pm, DC.pp, DC.v, v);
+        for (int sw = 0; sw < h->opts.nu_post; sw++) {
+            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
+            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 0, 0);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- setup
+
+int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcvg, int maxlevel,
+             const ngap_opts *o) {
+    if (!h) return NGAP_ERR_ARG;
+    if (!h->init_done) return NGAP_ERR_NOT_INIT;
+    if (!opr) return NGAP_ERR_ARG;
+    if (h->method != NGAP_PCG_PFMG) return NGAP_ERR_METHOD;
+    if (maxit < 0 || !(rcvg >= 0.0)) return NGAP_ERR_ARG;
+    double t0 = now_ms();
+    h->setup_done = false;
+    ngap_opts op;
+    ngap_default_opts(&op);
+    if (o) op = *o;
+    if (op.nu_pre < 1) op.nu_pre = 1;
+    if (op.nu_post < 1) op.nu_post = 1;
+    if (op.coarsest_max < 1) op.coarsest_max = 128;
+    if (op.precision != NGAP_FP64 && op.precision != NGAP_MIXED) return NGAP_ERR_ARG;
+    h->opts = op;
+    h->maxit = maxit;
+    h->rcvg = rcvg;
+    h->maxlevel = maxlevel;
+    HIPCHK(hipSetDevice(h->device));
+    int rc = ensure_base(h);
+    if (rc) return rc;
+    hipStream_t s = h->stream;
+    size_t obytes = sizeof(double) * (size_t)h->nghost * h->nst;
+    if (on_device) {
+        // the caller's producer may run on another stream: wait for the device first
+        HIPCHK(hipDeviceSynchronize());
+        HIPCHK(hipMemcpyAsync(h->opr, opr, obytes, hipMemcpyDeviceToDevice, s));
+    } else {
+        HIPCHK(hipMemcpyAsync(h->opr, opr, obytes, hipMemcpyHostToDevice, s));
+    }
+    // level 0
+    if (h->lev.empty()) {
+        h->lev.resize(1);
+        h->lev[0].g = h->g0;
+        if (alloc_lvl_data(h, h->lev[0].d, h->g0.N, false) || alloc_lvl_data(h, h->lev[0].s, h->g0.N, false))
+            return NGAP_ERR_ALLOC;
+    }
+    Level &L0 = h->lev[0];
+    int nb = grid_for(h->g0.N);
+    hipLaunchKernelGGL(k_level0, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots, (const double *)h->opr, L0.d.c,
+                       L0.d.a0, L0.d.a1, L0.d.a2, h->part, h->parts, h->parts + MAX_PARTS);
+    hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, (int)S_ASYM);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
+                       (int)S_BAD);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scal(h))) return rc;
+    h->st.asym = h->h_scal[S_ASYM];
+    if (op.check_finite && h->h_scal[S_BAD] > 0.0) return NGAP_ERR_NONFINITE;
+    // coarser levels
+    int nlev = 1;
+    while (true) {
+        Level &F = h->lev[nlev - 1];
+        F.cdir = -1;
+        if (maxlevel > 0 && nlev >= maxlevel) break;
+        if (F.g.N <= op.coarsest_max) break;
+        if (nlev >= 64) break;
+        int np = 0;
+        launch_strength(s, F.g, F.d.a0, F.d.a1, F.d.a2, h->parts, &np);
+        for (int d = 0; d < 3; d++)
+            hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + d * MAX_PARTS), np,
+                               h->scal, (int)(S_STR0 + d));
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scal(h))) return rc;
+        double str[3];
+        for (int d = 0; d < 3; d++) str[d] = can_coarsen(F.g, d) ? h->h_scal[S_STR0 + d] : -1.0;
+        int best = -1;
+        for (int d = 0; d < 3; d++) {
+            if (str[d] < 0) continue;
+            if (best < 0 || str[d] > str[best] * (1.0 + 1e-6)) best = d;
+        }
+        if (best < 0) break;
+        Geo cg = F.g;
+        int per = best == 0 ? F.g.p0 : (best == 1 ? F.g.p1 : F.g.p2);
+        int *nd = best == 0 ? &cg.n0 : (best == 1 ? &cg.n1 : &cg.n2);
+        *nd = per ? *nd / 2 : (*nd + 1) / 2;
+        cg.N = (long long)cg.n0 * cg.n1 * cg.n2;
+        F.cdir = best;
+        if ((int)h->lev.size() > nlev && h->lev[nlev].g.N != cg.N) {
+            for (size_t q = nlev; q < h->lev.size(); q++) free_level(h, h->lev[q]);
+            h->lev.resize(nlev);
+        }
+        if ((int)h->lev.size() == nlev) {
+            h->lev.emplace_back();
+            Level &C = h->lev.back();
+            C.g = cg;
+            if (alloc_lvl_data(h, C.d, cg.N, true) || alloc_lvl_data(h, C.s, cg.N, true)) return NGAP_ERR_ALLOC;
+        }
+        Level &Fr = h->lev[nlev - 1];  // reference may have moved
+        Level &C = h->lev[nlev];
+        C.g = cg;
+        C.cdir = -1;
+        launch_coarse_build(s, Fr.g, cg, best, Fr.d.c, Fr.d.a0, Fr.d.a1, Fr.d.a2, C.d.pm, C.d.pp, C.d.c, C.d.a0,
+                            C.d.a1, C.d.a2, C.d.f);
+        HIPCHK(hipGetLastError());
+        nlev++;
+    }
+    for (size_t q = nlev; q < h->lev.size(); q++) free_level(h, h->lev[q]);
+    h->lev.resize(nlev);
+    h->nlev = nlev;
+    HIPCHK(hipStreamSynchronize(s));
+    Level &C = h->lev[nlev - 1];
+    h->coarse_dense = C.g.N <= 1024;
+    h->coarse_singular = 0;
+    if (h->coarse_dense) {
+        if ((rc = build_coarsest_inverse(h))) return rc;
+    }
+    // FP32 copies of the hierarchy for the mixed-precision V-cycle
+    if (op.precision == NGAP_MIXED) {
+        for (int l = 0; l < nlev; l++) {
+            Level &L = h->lev[l];
+            launch_to_f32(s, L.g.N, L.d.c, L.s.c);
+            launch_to_f32(s, L.g.N, L.d.a0, L.s.a0);
+            launch_to_f32(s, L.g.N, L.d.a1, L.s.a1);
+            launch_to_f32(s, L.g.N, L.d.a2, L.s.a2);
+            if (l > 0) {
+                launch_to_f32(s, L.g.N, L.d.pm, L.s.pm);
+                launch_to_f32(s, L.g.N, L.d.pp, L.s.pp);
+            }
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    h->setup_done = true;
+    h->st.nlevels = nlev;
+    h->st.coarse_singular = h->coarse_singular;
+    h->st.setup_ms = now_ms() - t0;
+    h->st.cells = h->g0.N;
+    h->st.device_bytes = h->dev_bytes;
+    return 0;
+}
+
+// ---------------------------------------------------------------- PCG
+
+int exact_residual(ngap_t *h, double *out, int slot) {
+    int nb = grid_for(h->g0.N);
+    hipLaunchKernelGGL(k_exact_resid, dim3(nb), dim3(NT), 0, h->stream, h->g0, h->gh, h->slots,
+                       (const double *)h->opr, (const double *)h->b, (const double *)h->x, out, h->parts);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, h->stream, (const double *)h->parts, nb, h->scal, slot);
+    return 0;
+}
+
+void precond(ngap_t *h, bool mixed) {
+    if (mixed)
+        vcycle<float>(h, h->r32, h->z32);
+    else
+        vcycle<double>(h, h->r, h->z);
+}
+
+// r, z -> rho (restart: p = z)
+void restart_dir(ngap_t *h, bool mixed, bool first) {
+    long long N = h->g0.N;
+    int nb = grid_for(N);
+    precond(h, mixed);
+    if (mixed)
+        hipLaunchKernelGGL(k_dot_rz<float>, dim3(nb), dim3(NT), 0, h->stream, N, (const double *)h->r,
+                           (const float *)h->z32, (const double *)nullptr, h->parts, (double *)nullptr);
+    else
+        hipLaunchKernelGGL(k_dot_rz<double>, dim3(nb), dim3(NT), 0, h->stream, N, (const double *)h->r,
+                           (const double *)h->z, (const double *)nullptr, h->parts, (double *)nullptr);
+    hipLaunchKernelGGL(k_fin_rho, dim3(1), dim3(NT), 0, h->stream, (const double *)h->parts, nb, h->scal, h->flags,
+                       first ? 1 : 0);
+    if (mixed)
+        hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, h->stream, N, h->p, (const float *)h->z32,
+                           (const double *)h->scal, 1);
+    else
+        hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, h->stream, N, h->p, (const double *)h->z,
+                           (const double *)h->scal, 1);
+}
+
+int run_pcg(ngap_t *h) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    bool mixed = h->opts.precision == NGAP_MIXED;
+    const bool flexible = mixed;
+    int rc;
+    h->st.fallback = 0;
+    h->st.restarts = 0;
+    hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, h->flags);
+    exact_residual(h, h->r, S_RR);
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(NT), 0, s, N, (const double *)h->b, h->parts + MAX_PARTS);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
+                       (int)S_BB);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scal(h))) return rc;
+    const double bn = std::sqrt(h->h_scal[S_BB]);
+    if (bn == 0.0) {
+        hipLaunchKernelGGL(k_zero64, dim3(nb), dim3(NT), 0, s, N, h->x);
+        HIPCHK(hipGetLastError());
+        h->st.it = 0;
+        h->st.rerr = 0.0;
+        h->st.true_rerr = 0.0;
+        h->st.aerr = 0.0;
+        h->st.cnorm_ratio = 0.0;
+        h->st.status = NGAP_STATUS_CONVERGED;
+        return 0;
+    }
+    double rel = std::sqrt(h->h_scal[S_RR]) / bn;
+    const double tol = h->rcvg;
+    int it = 0;
+    bool restart = true, first = true, converged = false;
+    double tr = rel;
+    if (mixed) launch_to_f32(s, N, h->r, h->r32);
+    while (true) {
+        if (rel <= tol) {
+            exact_residual(h, h->q, S_RRT);
+            HIPCHK(hipGetLastError());
+            if ((rc = read_scal(h))) return rc;
+            tr = std::sqrt(h->h_scal[S_RRT]) / bn;
+            if (tr <= tol || it >= h->maxit) {
+                converged = tr <= tol;
+                break;
+            }
+            hipLaunchKernelGGL(k_copy, dim3(nb), dim3(NT), 0, s, N, (const double *)h->q, h->r);
+            if (mixed) launch_to_f32(s, N, h->r, h->r32);
+            rel = tr;
+            restart = true;
+            h->st.restarts++;
+        }
+        if (it >= h->maxit) break;
+        if (restart) {
+            restart_dir(h, mixed, first);
+            first = false;
+            restart = false;
+        }
+        Level &L0 = h->lev[0];
+        launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->p, h->q, h->parts);
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, h->flags);
+        hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
+                           (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
+                           h->parts + MAX_PARTS);
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
+                           (int)S_RR);
+        HIPCHK(hipGetLastError());
+        it++;
+        if ((rc = read_scal(h))) return rc;
+        rel = std::sqrt(h->h_scal[S_RR]) / bn;
+        if (h->h_flags[0]) {
+            // preconditioner or step breakdown: x, r untouched by a zero step
+            hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, h->flags);
+            if (mixed) {
+                mixed = false;
+                h->st.fallback = 1;
+                restart = true;
+                continue;
+            }
+            if (!std::isfinite(rel)) return NGAP_ERR_BREAKDOWN;
+            if (h->st.restarts > 2) return NGAP_ERR_BREAKDOWN;
+            h->st.restarts++;
+            restart = true;
+            continue;
+        }
+        if (!std::isfinite(rel)) return NGAP_ERR_BREAKDOWN;
+        if (rel <= tol || it >= h->maxit) continue;
+        precond(h, mixed);
+        if (mixed)
+            hipLaunchKernelGGL(k_dot_rz<float>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r,
+                               (const float *)h->z32, flexible ? (const double *)h->q : (const double *)nullptr,
+                               h->parts, h->parts + MAX_PARTS);
+        else
+            hipLaunchKernelGGL(k_dot_rz<double>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r,
+                               (const double *)h->z, flexible ? (const double *)h->q : (const double *)nullptr,
+                               h->parts, h->parts + MAX_PARTS);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, (const double *)h->parts,
+                           (const double *)(h->parts + MAX_PARTS), nb, h->scal, h->flags, flexible ? 1 : 0);
+        if (mixed)
+            hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, s, N, h->p, (const float *)h->z32,
+                               (const double *)h->scal, 0);
+        else
+            hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, s, N, h->p, (const double *)h->z,
+                               (const double *)h->scal, 0);
+    }
+    if (!converged && it >= h->maxit && rel > tol) {
+        exact_residual(h, h->q, S_RRT);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scal(h))) return rc;
+        tr = std::sqrt(h->h_scal[S_RRT]) / bn;
+    }
+    h->st.it = it;
+    h->st.rerr = rel;
+    h->st.true_rerr = tr;
+    h->st.aerr = tr * bn;
+    double rho0 = h->h_scal[S_RHO0], rho = h->h_scal[S_RHO];
+    h->st.cnorm_ratio = (rho0 > 0.0 && rho >= 0.0) ? std::sqrt(rho / rho0) : 0.0;
+    h->st.status = converged ? NGAP_STATUS_CONVERGED : NGAP_STATUS_MAXIT;
+    h->st.precision = mixed ? NGAP_MIXED : NGAP_FP64;
+    return 0;
+}
+
+int do_solve(ngap_t *h, const double *rhs, double *sol, bool on_device, int *it, double *rerr, double *aerr) {
+    if (!h || !rhs || !sol) return NGAP_ERR_ARG;
+    if (!h->setup_done) return NGAP_ERR_NOT_SETUP;
+    double t0 = now_ms();
+    HIPCHK(hipSetDevice(h->device));
+    hipStream_t s = h->stream;
+    const double *drhs = rhs;
+    double *dsol = sol;
+    double tc = 0.0;
+    size_t gb = sizeof(double) * (size_t)h->nghost;
+    if (on_device) {
+        // rhs/sol may have been written on another stream: wait for the device first
+        HIPCHK(hipDeviceSynchronize());
+    } else {
+        double c0 = now_ms();
+        HIPCHK(hipMemcpyAsync(h->rhs_g, rhs, gb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipMemcpyAsync(h->sol_g, sol, gb, hipMemcpyHostToDevice, s));
+        HIPCHK(hipStreamSynchronize(s));
+        tc += now_ms() - c0;
+        drhs = h->rhs_g;
+        dsol = h->sol_g;
+    }
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, drhs, (const unsigned char *)h->part, h->b,
+                       h->opts.check_finite ? h->parts : (double *)nullptr);
+    if (h->opts.check_finite)
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, (int)S_BAD);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, (const double *)dsol,
+                       (const unsigned char *)h->part, h->x, h->opts.check_finite ? h->parts + MAX_PARTS : nullptr);
+    if (h->opts.check_finite) {
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
+                           (int)S_RRT);
+        HIPCHK(hipGetLastError());
+        int rc = read_scal(h);
+        if (rc) return rc;
+        if (h->h_scal[S_BAD] > 0.0 || h->h_scal[S_RRT] > 0.0) return NGAP_ERR_NONFINITE;
+    }
+    HIPCHK(hipGetLastError());
+    int rc = run_pcg(h);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, (const double *)h->x, dsol);
+    HIPCHK(hipGetLastError());
+    if (!on_device) {
+        double c0 = now_ms();
+        HIPCHK(hipMemcpyAsync(sol, h->sol_g, gb, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        tc += now_ms() - c0;
+    } else {
+        HIPCHK(hipStreamSynchronize(s));
+    }
+    h->st.copy_ms = tc;
+    h->st.solve_ms = now_ms() - t0;
+    if (it) *it = h->st.it;
+    if (rerr) *rerr = h->st.rerr;
+    if (aerr) *aerr = h->st.aerr;
+    if (h->opts.verbose > 0)
+        std::fprintf(stderr, "[ngap] it=%d rerr=%.3e true=%.3e levels=%d %s%s solve=%.3f ms\n", h->st.it,
+                     h->st.rerr, h->st.true_rerr, h->st.nlevels, h->st.precision == NGAP_MIXED ? "mixed" : "fp64",
+                     h->st.fallback ? " (fallback)" : "", h->st.solve_ms);
+    return 0;
+}
+
+// Device staging for the test hooks
+struct Tmp {
+    std::vector<std::pair<void *, size_t>> v;
+    ~Tmp() {
+        for (auto &q : v) (void)hipFree(q.first);
+    }
+    template <typename T>
+    T *get(size_t n) {
+        void *p = nullptr;
+        if (hipMalloc(&p, sizeof(T) * (n ? n : 1)) != hipSuccess) return nullptr;
+        v.push_back({p, sizeof(T) * n});
+        return (T *)p;
+    }
+};
+
+template <typename T>
+int up(hipStream_t s, Tmp &tmp, const double *src, size_t n, T **out) {
+    double *d = tmp.get<double>(n);
+    if (!d) return NGAP_ERR_ALLOC;
+    HIPCHK(hipMemcpyAsync(d, src, sizeof(double) * n, hipMemcpyHostToDevice, s));
+    if constexpr (std::is_same<T, double>::value) {
+        *out = d;
+    } else {
+        float *f = tmp.get<float>(n);
+        if (!f) return NGAP_ERR_ALLOC;
+        launch_to_f32(s, (long long)n, d, f);
+        *out = f;
+    }
+    return 0;
+}
+
+template <typename T>
+int down(hipStream_t s, Tmp &tmp, const T *src, size_t n, double *dst) {
+    const double *d;
+    if constexpr (std::is_same<T, double>::value) {
+        d = src;
+    } else {
+        double *t = tmp.get<double>(n);
+        if (!t) return NGAP_ERR_ALLOC;
+        launch_to_f64(s, (long long)n, src, t);
+        d = t;
+    }
+    HIPCHK(hipMemcpyAsync(dst, d, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace
+
+namespace {
+
+template <typename T>
+static int t_apply(ngap_t *h, int l, const double *x, double *y) {
+    Tmp tmp;
+    Level &L = h->lev[l];
+    LevelData<T> &D = data<T>(L);
+    size_t n = (size_t)L.g.N;
+    T *dx, *dy = tmp.get<T>(n);
+    int rc = up<T>(h->stream, tmp, x, n, &dx);
+    if (rc || !dy) return rc ? rc : NGAP_ERR_ALLOC;
+    launch_apply<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, dx, dy, nullptr);
+    HIPCHK(hipGetLastError());
+    return down<T>(h->stream, tmp, dy, n, y);
+}
+
+template <typename T>
+static int t_rbgs(ngap_t *h, int l, const double *f, double *v, int order) {
+    Tmp tmp;
+    Level &L = h->lev[l];
+    LevelData<T> &D = data<T>(L);
+    size_t n = (size_t)L.g.N;
+    T *df, *dv;
+    int rc = up<T>(h->stream, tmp, f, n, &df);
+    if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
+    if (rc) return rc;
+    int c1 = order ? 1 : 0;
+    launch_rbgs<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, c1, 0);
+    launch_rbgs<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, 1 - c1, 0);
+    HIPCHK(hipGetLastError());
+    return down<T>(h->stream, tmp, dv, n, v);
+}
+
+template <typename T>
+static int t_rr(ngap_t *h, int l, const double *f, const double *v, double *fc) {
+    Tmp tmp;
+    Level &L = h->lev[l];
+    Level &C = h->lev[l + 1];
+    LevelData<T> &D = data<T>(L);
+    LevelData<T> &DC = data<T>(C);
+    size_t n = (size_t)L.g.N, nc = (size_t)C.g.N;
+    T *df, *dv, *dfc = tmp.get<T>(nc);
+    int rc = up<T>(h->stream, tmp, f, n, &df);
+    if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
+    if (rc || !dfc) return rc ? rc : NGAP_ERR_ALLOC;
+    launch_resid_restrict<T>(h->stream, L.g, C.g, L.cdir, D.c, D.a0, D.a1, D.a2, df, dv, DC.pm, DC.pp, dfc);
+    HIPCHK(hipGetLastError());
+    return down<T>(h->stream, tmp, dfc, nc, fc);
+}
+
+template <typename T>
+static int t_pr(ngap_t *h, int l, const double *vc, double *v) {
+    Tmp tmp;
+    Level &L = h->lev[l];
+    Level &C = h->lev[l + 1];
+    LevelData<T> &DC = data<T>(C);
+    size_t n = (size_t)L.g.N, nc = (size_t)C.g.N;
+    T *dvc, *dv;
+    int rc = up<T>(h->stream, tmp, vc, nc, &dvc);
+    if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
+    if (rc) return rc;
+    launch_prolong<T>(h->stream, L.g, C.g, L.cdir, DC.pm, DC.pp, dvc, dv);
+    HIPCHK(hipGetLastError());
+    return down<T>(h->stream, tmp, dv, n, v);
+}
+
+template <typename T>
+static int t_cs(ngap_t *h, const double *f, double *v) {
+    Tmp tmp;
+    Level &C = h->lev[h->nlev - 1];
+    size_t n = (size_t)C.g.N;
+    T *df, *dv = tmp.get<T>(n);
+    int rc = up<T>(h->stream, tmp, f, n, &df);
+    if (rc || !dv) return rc ? rc : NGAP_ERR_ALLOC;
+    coarse_solve<T>(h, df, dv);
+    HIPCHK(hipGetLastError());
+    return down<T>(h->stream, tmp, dv, n, v);
+}
+
+template <typename T>
+static int t_vc(ngap_t *h, const double *r, double *z) {
+    Tmp tmp;
+    size_t n = (size_t)h->g0.N;
+    T *dr, *dz = tmp.get<T>(n);
+    int rc = up<T>(h->stream, tmp, r, n, &dr);
+    if (rc || !dz) return rc ? rc : NGAP_ERR_ALLOC;
+    vcycle<T>(h, dr, dz);
+    HIPCHK(hipGetLastError());
+    return down<T>(h->stream, tmp, dz, n, z);
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+
+extern "C" {
+
+int ngap_default_opts(ngap_opts *o) {
+    if (!o) return NGAP_ERR_ARG;
+    o->precision = NGAP_FP64;
+    o->nu_pre = 1;
+    o->nu_post = 1;
+    o->coarsest_max = 128;
+    o->check_finite = 1;
+    o->verbose = 0;
+    return 0;
+}
+
+const char *ngap_strerror(int code) {
+    switch (code) {
+        case NGAP_OK: return "ok";
+        case NGAP_ERR_ARG: return "invalid argument";
+        case NGAP_ERR_STENCIL: return "stencil is not the 7-point set";
+        case NGAP_ERR_NOT_INIT: return "solver has not been initialised";
+        case NGAP_ERR_NOT_SETUP: return "solver has not been setup";
+        case NGAP_ERR_METHOD: return "unknown or unsupported solution method";
+        case NGAP_ERR_UNSUPPORTED: return "periodic direction with an odd number of cells";
+        case NGAP_ERR_HIP: return "HIP runtime error";
+        case NGAP_ERR_NONFINITE: return "non-finite value in opr, rhs or initial guess";
+        case NGAP_ERR_BREAKDOWN: return "PCG breakdown";
+        case NGAP_ERR_ALLOC: return "allocation failed";
+        case NGAP_ERR_COARSE: return "coarsest-level factorisation failed";
+        default: return "unknown error";
+    }
+}
+
+int ngap_create(ngap_t **hp, const ngap_grid *g, int nst, const int *stc, int method) {
+    if (!hp) return NGAP_ERR_ARG;
+    *hp = nullptr;
+    if (!g || !stc || nst < 1) return NGAP_ERR_ARG;
+    if (g->nx < 1 || g->ny < 1 || g->nz < 1) return NGAP_ERR_ARG;
+    // one GPU holds the whole grid: local bounds must span the global grid
+    if (g->imax_ - g->imin_ + 1 != g->nx || g->jmax_ - g->jmin_ + 1 != g->ny || g->kmax_ - g->kmin_ + 1 != g->nz)
+        return NGAP_ERR_ARG;
+    if (g->imino_ > g->imin_ || g->imaxo_ < g->imax_ || g->jmino_ > g->jmin_ || g->jmaxo_ < g->jmax_ ||
+        g->kmino_ > g->kmin_ || g->kmaxo_ < g->kmax_)
+        return NGAP_ERR_ARG;
+    ngap_t *h = new (std::nothrow) ngap_t();
+    if (!h) return NGAP_ERR_ALLOC;
+    h->g = *g;
+    h->nst = nst;
+    h->stc.assign(stc, stc + 3 * nst);
+    h->method = method;
+    h->st.it = 0;
+    ngap_default_opts(&h->opts);
+    *hp = h;
+    return 0;
+}
+
+int ngap_init(ngap_t *h) {
+    if (!h) return NGAP_ERR_ARG;
+    // inverse stencil map (hypre_str_class.f90:116-123): slot of each 7-point offset
+    const int want[7][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+    if (h->nst != 7) return NGAP_ERR_STENCIL;
+    for (int w = 0; w < 7; w++) {
+        int found = -1;
+        for (int st = 0; st < h->nst; st++) {
+            int dx = h->stc[st], dy = h->stc[h->nst + st], dz = h->stc[2 * h->nst + st];
+            if (dx == want[w][0] && dy == want[w][1] && dz == want[w][2]) {
+                if (found >= 0) return NGAP_ERR_STENCIL;
+                found = st;
+            }
+        }
+        if (found < 0) return NGAP_ERR_STENCIL;
+        h->slots.s[w] = found;
+    }
+    h->slots.nst = h->nst;
+    const ngap_grid &g = h->g;
+    const int n3[3] = {g.nx, g.ny, g.nz}, p3[3] = {g.xper != 0, g.yper != 0, g.zper != 0};
+    for (int d = 0; d < 3; d++)
+        if (p3[d] && n3[d] >= 3 && (n3[d] & 1)) return NGAP_ERR_UNSUPPORTED;
+    h->g0.n0 = g.nx;
+    h->g0.n1 = g.ny;
+    h->g0.n2 = g.nz;
+    h->g0.p0 = p3[0];
+    h->g0.p1 = p3[1];
+    h->g0.p2 = p3[2];
+    h->g0.N = (long long)g.nx * g.ny * g.nz;
+    h->gh.nxo = g.imaxo_ - g.imino_ + 1;
+    h->gh.nyo = g.jmaxo_ - g.jmino_ + 1;
+    long long nzo = g.kmaxo_ - g.kmino_ + 1;
+    h->gh.oi = g.imin_ - g.imino_;
+    h->gh.oj = g.jmin_ - g.jmino_;
+    h->gh.ok = g.kmin_ - g.kmino_;
+    h->nghost = h->gh.nxo * h->gh.nyo * nzo;
+    if (!h->init_done) {
+        if (hipGetDevice(&h->device) != hipSuccess) return NGAP_ERR_HIP;
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return NGAP_ERR_HIP;
+    }
+    h->init_done = true;
+    return 0;
+}
+
+int ngap_setup(ngap_t *h, const double *opr, int maxit, double rcvg, int maxlevel, const ngap_opts *o) {
+    return do_setup(h, opr, false, maxit, rcvg, maxlevel, o);
+}
+
+int ngap_setup_device(ngap_t *h, const double *d_opr, int maxit, double rcvg, int maxlevel, const ngap_opts *o) {
+    return do_setup(h, d_opr, true, maxit, rcvg, maxlevel, o);
+}
+
+int ngap_solve(ngap_t *h, const double *rhs, double *sol, int *it, double *rerr, double *aerr) {
+    return do_solve(h, rhs, sol, false, it, rerr, aerr);
+}
+
+int ngap_solve_device(ngap_t *h, const double *d_rhs, double *d_sol, int *it, double *rerr, double *aerr) {
+    return do_solve(h, d_rhs, d_sol, true, it, rerr, aerr);
+}
+
+int ngap_set_tolerance(ngap_t *h, int maxit, double rcvg) {
+    if (!h || maxit < 0 || !(rcvg >= 0.0)) return NGAP_ERR_ARG;
+    h->maxit = maxit;
+    h->rcvg = rcvg;
+    return 0;
+}
+
+int ngap_destroy(ngap_t *h) {
+    if (!h) return NGAP_ERR_ARG;
+    if (h->init_done) {
+        (void)hipSetDevice(h->device);
+        if (h->stream) (void)hipStreamSynchronize(h->stream);
+        free_all(h);
+        if (h->h_scal) (void)hipHostFree(h->h_scal);
+        if (h->h_flags) (void)hipHostFree(h->h_flags);
+        if (h->stream) (void)hipStreamDestroy(h->stream);
+    }
+    delete h;
+    return 0;
+}
+
+int ngap_last_stats(const ngap_t *h, ngap_stats *s) {
+    if (!h || !s) return NGAP_ERR_ARG;
+    *s = h->st;
+    s->device_bytes = h->dev_bytes;
+    return 0;
+}
+
+// ------------------------------------------------------------ test hooks
+
+int ngapt_nlevels(const ngap_t *h) { return (h && h->setup_done) ? h->nlev : -1; }
+
+int ngapt_level_info(const ngap_t *h, int l, int *dims3, int *cdir) {
+    if (!h || !h->setup_done || l < 0 || l >= h->nlev || !dims3 || !cdir) return NGAP_ERR_ARG;
+    dims3[0] = h->lev[l].g.n0;
+    dims3[1] = h->lev[l].g.n1;
+    dims3[2] = h->lev[l].g.n2;
+    *cdir = h->lev[l].cdir;
+    return 0;
+}
+
+int ngapt_level_coefs(const ngap_t *h, int l, double *c, double *ax, double *ay, double *az, double *pm, double *pp) {
+    if (!h || !h->setup_done || l < 0 || l >= h->nlev) return NGAP_ERR_ARG;
+    const Level &L = h->lev[l];
+    size_t b = sizeof(double) * (size_t)L.g.N;
+    if (c) HIPCHK(hipMemcpy(c, L.d.c, b, hipMemcpyDeviceToHost));
+    if (ax) HIPCHK(hipMemcpy(ax, L.d.a0, b, hipMemcpyDeviceToHost));
+    if (ay) HIPCHK(hipMemcpy(ay, L.d.a1, b, hipMemcpyDeviceToHost));
+    if (az) HIPCHK(hipMemcpy(az, L.d.a2, b, hipMemcpyDeviceToHost));
+    if (l > 0) {
+        if (pm) HIPCHK(hipMemcpy(pm, L.d.pm, b, hipMemcpyDeviceToHost));
+        if (pp) HIPCHK(hipMemcpy(pp, L.d.pp, b, hipMemcpyDeviceToHost));
+    } else {
+        if (pm) std::memset(pm, 0, b);
+        if (pp) std::memset(pp, 0, b);
+    }
+    return 0;
+}
+
+#define TEST_GUARD(l)                                                                   \
+    if (!h || !h->setup_done || (l) < 0 || (l) >= h->nlev) return NGAP_ERR_ARG;        \
+    if (prec == 1 && h->opts.precision != NGAP_MIXED) return NGAP_ERR_ARG;              \
+    if (prec != 0 && prec != 1) return NGAP_ERR_ARG;                                    \
+    HIPCHK(hipSetDevice(h->device));
+
+int ngapt_apply(ngap_t *h, int l, int prec, const double *x, double *y) {
+    TEST_GUARD(l);
+    return prec ? t_apply<float>(h, l, x, y) : t_apply<double>(h, l, x, y);
+}
+
+int ngapt_rbgs(ngap_t *h, int l, int prec, const double *f, double *v, int order) {
+    TEST_GUARD(l);
+    return prec ? t_rbgs<float>(h, l, f, v, order) : t_rbgs<double>(h, l, f, v, order);
+}
+
+int ngapt_resid_restrict(ngap_t *h, int l, int prec, const double *f, const double *v, double *fc) {
+    TEST_GUARD(l + 1);
+    return prec ? t_rr<float>(h, l, f, v, fc) : t_rr<double>(h, l, f, v, fc);
+}
+
+int ngapt_prolong_add(ngap_t *h, int l, int prec, const double *vc, double *v) {
+    TEST_GUARD(l + 1);
+    return prec ? t_pr<float>(h, l, vc, v) : t_pr<double>(h, l, vc, v);
+}
+
+int ngapt_coarse_solve(ngap_t *h, int prec, const double *f, double *v) {
+    TEST_GUARD(0);
+    return prec ? t_cs<float>(h, f, v) : t_cs<double>(h, f, v);
+}
+
+int ngapt_vcycle(ngap_t *h, int prec, const double *r, double *z) {
+    TEST_GUARD(0);
+    return prec ? t_vc<float>(h, r, z) : t_vc<double>(h, r, z);
+}
+
+int ngapt_apply_exact(ngap_t *h, const double *x, double *y) {
+    if (!h || !h->setup_done || !x || !y) return NGAP_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    size_t n = (size_t)h->g0.N;
+    // y = A x  computed as b - (b - A x) with b = 0 in the q buffer
+    hipLaunchKernelGGL(k_zero64, dim3(grid_for(n)), dim3(NT), 0, h->stream, (long long)n, h->b);
+    HIPCHK(hipMemcpyAsync(h->x, x, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    exact_residual(h, h->q, S_RRT);
+    HIPCHK(hipGetLastError());
+    std::vector<double> t(n);
+    HIPCHK(hipMemcpyAsync(t.data(), h->q, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (size_t i = 0; i < n; i++) y[i] = -t[i];
+    return 0;
+}
+
+int ngapt_dot(ngap_t *h, long long n, const double *x, const double *y, double *out) {
+    if (!h || !h->setup_done || n < 0 || !x || !y || !out) return NGAP_ERR_ARG;
+    HIPCHK(hipSetDevice(h->device));
+    Tmp tmp;
+    double *dx = tmp.get<double>(n), *dy = tmp.get<double>(n);
+    if (!dx || !dy) return NGAP_ERR_ALLOC;
+    HIPCHK(hipMemcpyAsync(dx, x, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(dy, y, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    int nb = grid_for(n);
+    hipLaunchKernelGGL((k_dot<double, double>), dim3(nb), dim3(NT), 0, h->stream, n, (const double *)dx,
+                       (const double *)dy, h->parts);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, h->stream, (const double *)h->parts, nb, h->scal, (int)S_RRT);
+    HIPCHK(hipGetLastError());
+    int rc = read_scal(h);
+    if (rc) return rc;
+    *out = h->h_scal[S_RRT];
+    return 0;
+}
+
+int ngapt_time_kernel(ngap_t *h, int kid, int prec, int reps, double *avg_ms, double *bytes) {
+    TEST_GUARD(0);
+    if (reps < 1 || !avg_ms || !bytes) return NGAP_ERR_ARG;
+    hipStream_t s = h->stream;
+    Level &L0 = h->lev[0];
+    const long long N = L0.g.N;
+    const double w = prec ? 4.0 : 8.0;
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    auto launch = [&]() -> int {
+        switch (kid) {
+            case NGAPT_K_SPMV:
+                if (prec)
+                    launch_apply<float>(s, L0.g, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->z32, h->r32, nullptr);
+                else
+                    launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->p, h->q, h->parts);
+                return 0;
+            case NGAPT_K_RBGS:
+                if (prec)
+                    launch_rbgs<float>(s, L0.g, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32, h->z32, 0, 0);
+                else
+                    launch_rbgs<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r, h->z, 0, 0);
+                return 0;
+            case NGAPT_K_RESTRICT: {
+                if (h->nlev < 2) return NGAP_ERR_ARG;
+                Level &C = h->lev[1];
+                if (prec)
+                    launch_resid_restrict<float>(s, L0.g, C.g, L0.cdir, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32,
+                                                 h->z32, C.s.pm, C.s.pp, C.s.f);
+                else
+                    launch_resid_restrict<double>(s, L0.g, C.g, L0.cdir, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r,
+                                                  h->z, C.d.pm, C.d.pp, C.d.f);
+                return 0;
+            }
+            case NGAPT_K_PROLONG: {
+                if (h->nlev < 2) return NGAP_ERR_ARG;
+                Level &C = h->lev[1];
+                if (prec)
+                    launch_prolong<float>(s, L0.g, C.g, L0.cdir, C.s.pm, C.s.pp, C.s.v, h->z32);
+                else
+                    launch_prolong<double>(s, L0.g, C.g, L0.cdir, C.d.pm, C.d.pp, C.d.v, h->z);
+                return 0;
+            }
+            case NGAPT_K_UPDATE: {
+                int nb = grid_for(N);
+                hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
+                                   (const double *)h->q, (const double *)h->scal, (float *)nullptr, h->parts);
+                return 0;
+            }
+            case NGAPT_K_EXACT:
+                return exact_residual(h, h->q, S_RRT);
+            default:
+                return NGAP_ERR_ARG;
+        }
+    };
+    // byte model: every array touched once per launch
+    double b;
+    switch (kid) {
+        case NGAPT_K_SPMV: b = 6.0 * w * N; break;
+        case NGAPT_K_RBGS: b = 3.5 * w * N; break;  // one colour: half of (4 coef + f + v in + v out)
+        case NGAPT_K_RESTRICT: b = (h->nlev > 1 ? (6.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
+        case NGAPT_K_PROLONG: b = (h->nlev > 1 ? (2.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
+        case NGAPT_K_UPDATE: b = 6.0 * 8.0 * N; break;
+        case NGAPT_K_EXACT: b = (7.0 + 3.0) * 8.0 * N; break;
+        default: return NGAP_ERR_ARG;
+    }
+    int rc = launch();  // warm-up
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; r++) launch();
+    HIPCHK(hipEventRecord(e1, s));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *avg_ms = ms / reps;
+    *bytes = b;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+"""CPU checks of the boundary: the shared library loads and exports every
+symbol the headers declare; host-side mirrors (config, assembly) hold the
+invariants of NGA2's formulas. No kernel is launched here."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from nga2_amd import _lib, incomp, inputs
+from nga2_amd.config import Config, tanh_stretched, uniform
+
+
+def _declared(header):
+    txt = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(ngapt?_\w+)\s*\(", txt, re.M)))
+
+
+def test_library_exports_all_declared_symbols():
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = _declared("ngapsolve.h") + _declared("ngapsolve_test.h")
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_lib.EXPORTS) <= set(names)
+    assert set(_lib.TEST_EXPORTS) <= set(names)
+
+
+def test_strerror_and_defaults_without_gpu():
+    assert _lib.strerror(0) == "ok"
+    assert "setup" in _lib.strerror(_lib.ERR_NOT_SETUP)
+    o = _lib.default_opts()
+    assert (o.precision, o.nu_pre, o.nu_post, o.check_finite) == (0, 1, 1, 1)
+
+
+def test_create_rejects_bad_arguments_without_gpu():
+    """ngap_create validates before touching the device."""
+    cfg = Config(uniform(8), uniform(8), uniform(8), no=1)
+    from nga2_amd.linsol import grid_of
+
+    g = grid_of(cfg)
+    h = ctypes.c_void_p()
+    stc = np.asfortranarray(incomp.STC7)
+    L = _lib.lib()
+    assert L.ngap_create(ctypes.byref(h), ctypes.byref(g), 0, stc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 2) \
+        == _lib.ERR_ARG
+    g.imax_ = 5  # a partial box: this build holds the whole grid on one GPU
+    assert L.ngap_create(ctypes.byref(h), ctypes.byref(g), 7, stc.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), 2) \
+        == _lib.ERR_ARG
+    assert L.ngap_solve(None, None, None, None, None, None) == _lib.ERR_ARG
+
+
+def test_metrics_match_sgrid_formulas():
+    y = tanh_stretched(12, 1.0, 2.0)
+    cfg = Config(uniform(8), y, uniform(4), no=2, xper=True, yper=False, zper=True)
+    no = 2
+    # ghost extension by the first/last interior spacing (sgrid_class.f90:166-189)
+    assert np.isclose(cfg.y[0], y[0] - 2 * (y[1] - y[0]))
+    assert np.isclose(cfg.y[-1], y[-1] + 2 * (y[-1] - y[-2]))
+    assert np.allclose(cfg.dy[no:no + 12], np.diff(y))
+    assert np.allclose(cfg.dymi[1:], 1.0 / np.diff(cfg.ym))
+    assert np.allclose(cfg.vol[2, :, 3], cfg.dx[2] * cfg.dy * cfg.dz[3])
+
+
+def test_operator_closed_form_uniform():
+    """Uniform periodic grid: diagonal 6h, couplings -h (vol = h^3)."""
+    n = 16
+    P = inputs.periodic_gaussian(n=n)
+    it = P.cfg.interior
+    op = P.opr[(slice(None),) + it]
+    h = 1.0 / n
+    assert np.allclose(op[0], 6 * h, rtol=1e-13)
+    assert np.allclose(op[1:], -h, rtol=1e-13)
+
+
+def test_operator_invariants_channel_and_spheres():
+    for P in (inputs.channel_spectral(n=16), inputs.spheres_two_phase(n=16, nsph=4, rmin=0.15, rmax=0.3)):
+        cfg = P.cfg
+        it = cfg.interior
+        op = P.opr[(slice(None),) + it]
+        # row sums vanish (periodic/Neumann), centre positive, couplings negative
+        assert np.abs(op.sum(axis=0)).max() <= 1e-13 * op[0].max()
+        assert np.all(op[0] > 0)
+        assert np.all(op[1:] <= 0)
+        # symmetry opr(2,i) == opr(3,i+1) (x periodic in both workloads)
+        assert np.allclose(op[1], np.roll(op[2], -1, axis=0), rtol=1e-12, atol=0)
+        # rhs mean removed
+        assert abs(P.rhs[it].sum()) <= 1e-12 * np.abs(P.rhs[it]).sum()
+    # channel: Neumann walls in y -> no coupling across the wall
+    P = inputs.channel_spectral(n=16)
+    op = P.opr[(slice(None),) + P.cfg.interior]
+    assert np.all(op[4][:, 0, :] == 0.0) and np.all(op[3][:, -1, :] == 0.0)
+
+
+def test_shift_p_and_sync():
+    P = inputs.periodic_gaussian(n=8)
+    cfg = P.cfg
+    x = cfg.field()
+    x[cfg.interior] = np.random.default_rng(0).standard_normal((8, 8, 8)) + 3.0
+    incomp.shift_p(cfg, x)
+    assert abs(cfg.integrate(x)) < 1e-12
+    # periodic ghosts equal the wrapped interior
+    assert np.array_equal(x[0, 1:-1, 1:-1], x[8, 1:-1, 1:-1])
+    assert np.array_equal(x[9, 1:-1, 1:-1], x[1, 1:-1, 1:-1])
+
+
+def test_divergence_of_gradient_is_operator():
+    """-vol * div(grad p) from get_div(get_pgrad(p)) equals opr applied to p
+    (incomp_class.f90:766-787 vs :1144-1184)."""
+    P = inputs.channel_spectral(n=12)
+    cfg = P.cfg
+    met = incomp.Metrics(cfg)
+    p = cfg.field()
+    p[cfg.interior] = np.random.default_rng(2).standard_normal((12, 12, 12))
+    cfg.sync(p)
+    gx, gy, gz = incomp.get_pgrad(met, p)
+    div = incomp.get_div(met, gx, gy, gz)
+    lhs = (-cfg.vol * div)[cfg.interior]
+    from oracle.orc import Oracle, interior
+
+    o = Oracle.from_cfg(cfg, P.opr)
+    Ap = o.apply_exact(interior(p, cfg))
+    assert np.allclose(np.transpose(lhs, (2, 1, 0)).ravel(), Ap, rtol=1e-10, atol=1e-12 * np.abs(Ap).max())
