@@ -93,6 +93,7 @@ typedef struct ngap_opts {
     int coarsest_max; /* stop coarsening at <= this many cells, default 128 */
     int check_finite; /* scan opr/rhs/guess for NaN/Inf, default 1 */
     int verbose;      /* 0 quiet, 1 one line per solve on stderr */
+    int graphs;       /* replay the V-cycle as a HIP graph, default 1 */
 } ngap_opts;
 
 typedef struct ngap_stats {

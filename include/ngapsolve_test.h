@@ -20,8 +20,8 @@ extern "C" {
 
 /* kernel ids for ngapt_time_kernel */
 #define NGAPT_K_SPMV 0          /* fine-level 7-point SpMV (+ fused dot in FP64) */
-#define NGAPT_K_RBGS 1          /* fine-level full red+black Gauss-Seidel sweep */
-#define NGAPT_K_RESTRICT 2      /* fine-level residual + restriction to level 1 */
+#define NGAPT_K_RBGS 1          /* fine-level fused red+black Gauss-Seidel sweep (one launch) */
+#define NGAPT_K_RESTRICT 2      /* fine-level residual, then restriction to level 1 (two launches) */
 #define NGAPT_K_PROLONG 3       /* level 1 -> fine prolongation + correction */
 #define NGAPT_K_UPDATE 4        /* PCG x += alpha p, r -= alpha q, ||r||^2 */
 #define NGAPT_K_EXACT 5         /* residual with the exact 7-array operator */
@@ -30,6 +30,8 @@ int ngapt_nlevels(const ngap_t *h);
 int ngapt_level_info(const ngap_t *h, int l, int *dims3, int *cdir);
 int ngapt_level_coefs(const ngap_t *h, int l, double *c, double *ax, double *ay, double *az, double *pm, double *pp);
 int ngapt_apply(ngap_t *h, int l, int prec, const double *x, double *y);
+/* one symmetric Gauss-Seidel sweep; order bit 0: colour updated first
+ * (0 red, 1 black); bit 1: start from a zero iterate (v ignored on input) */
 int ngapt_rbgs(ngap_t *h, int l, int prec, const double *f, double *v, int order);
 int ngapt_resid_restrict(ngap_t *h, int l, int prec, const double *f, const double *v, double *fc);
 int ngapt_prolong_add(ngap_t *h, int l, int prec, const double *vc, double *v);

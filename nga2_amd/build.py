@@ -16,7 +16,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "lib")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["ngap_level.hip", "ngap_api.hip"]
+HIP_SOURCES = ["ngap_level.hip", "ngap_smooth.hip", "ngap_api.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
     # packed FP32 instructions stay off (they have given silent wrong results on this pool)

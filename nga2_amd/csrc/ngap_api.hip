@@ -368,6 +368,8 @@ struct ngap_t {
     double *Ainv = nullptr;
     int nc = 0;
     int coarse_dense = 0, coarse_singular = 0;
+    int apply_parts = 0;  // partial sums written by the fine-level SpMV
+    hipGraphExec_t vc_graph[2] = {nullptr, nullptr};  // V-cycle per precision (0 FP64, 1 FP32)
     ngap_stats st{};
     long long dev_bytes = 0;
     std::vector<void *> base_allocs;
@@ -406,17 +408,20 @@ void dfree(ngap_t *h, void *ptr, size_t bytes) {
     }
 }
 
+// coefficients and the second iterate buffer on every level; transfer
+// weights, right-hand side and final iterate on coarse levels only
 template <typename T>
-int alloc_lvl_data(ngap_t *h, LevelData<T> &D, long long N, bool work) {
+int alloc_lvl_data(ngap_t *h, LevelData<T> &D, long long N, bool coarse) {
     size_t b = sizeof(T) * (size_t)N;
     int rc = 0;
     rc |= dmalloc(h, (void **)&D.c, b);
     rc |= dmalloc(h, (void **)&D.a0, b);
     rc |= dmalloc(h, (void **)&D.a1, b);
     rc |= dmalloc(h, (void **)&D.a2, b);
-    rc |= dmalloc(h, (void **)&D.pm, b);
-    rc |= dmalloc(h, (void **)&D.pp, b);
-    if (work) {
+    rc |= dmalloc(h, (void **)&D.w, b);
+    if (coarse) {
+        rc |= dmalloc(h, (void **)&D.pm, b);
+        rc |= dmalloc(h, (void **)&D.pp, b);
         rc |= dmalloc(h, (void **)&D.f, b);
         rc |= dmalloc(h, (void **)&D.v, b);
     }
@@ -426,14 +431,7 @@ int alloc_lvl_data(ngap_t *h, LevelData<T> &D, long long N, bool work) {
 template <typename T>
 void free_lvl_data(ngap_t *h, LevelData<T> &D, long long N) {
     size_t b = sizeof(T) * (size_t)N;
-    dfree(h, D.c, b);
-    dfree(h, D.a0, b);
-    dfree(h, D.a1, b);
-    dfree(h, D.a2, b);
-    dfree(h, D.pm, b);
-    dfree(h, D.pp, b);
-    dfree(h, D.f, b);
-    dfree(h, D.v, b);
+    for (T *q : {D.c, D.a0, D.a1, D.a2, D.w, D.pm, D.pp, D.f, D.v}) dfree(h, q, b);
     D = LevelData<T>();
 }
 
@@ -442,7 +440,10 @@ void free_level(ngap_t *h, Level &L) {
     free_lvl_data(h, L.s, L.g.N);
 }
 
+void destroy_graphs(ngap_t *h);
+
 void free_all(ngap_t *h) {
+    destroy_graphs(h);
     for (auto &L : h->lev) free_level(h, L);
     h->lev.clear();
     h->nlev = 0;
@@ -510,7 +511,7 @@ int read_scal(ngap_t *h) {
 int can_coarsen(const Geo &g, int d) {
     int n = d == 0 ? g.n0 : (d == 1 ? g.n1 : g.n2);
     int per = d == 0 ? g.p0 : (d == 1 ? g.p1 : g.p2);
-    if (per) return (n % 2 == 0) && n >= 4;
+    if (per) return (n % 4) == 0;  // coarse periodic length stays even (red/black colouring)
     return n >= 2;
 }
 
@@ -611,6 +612,8 @@ LevelData<double> &data<double>(Level &L) { return L.d; }
 template <>
 LevelData<float> &data<float>(Level &L) { return L.s; }
 
+// Coarsest level: dense (pseudo-)inverse, or 8 RB/BR sweep pairs from zero.
+// The result lands in v.
 template <typename T>
 void coarse_solve(ngap_t *h, const T *f, T *v) {
     Level &C = h->lev[h->nlev - 1];
@@ -618,31 +621,48 @@ void coarse_solve(ngap_t *h, const T *f, T *v) {
     if (h->coarse_dense) {
         launch_dense<T, T>(h->stream, (int)C.g.N, h->Ainv, f, v);
     } else {
-        for (int s = 0; s < 8; s++) {
-            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 0, s == 0);
-            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
-            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
-            launch_rbgs<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, 0, 0);
+        launch_sweep<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, nullptr, D.w, 0);
+        launch_sweep<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, D.w, v, 1);
+        for (int s = 1; s < 8; s++) {
+            launch_sweep<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, v, D.w, 0);
+            launch_sweep<T>(h->stream, C.g, D.c, D.a0, D.a1, D.a2, f, D.w, v, 1);
         }
     }
 }
 
+// One V(nu_pre, nu_post) cycle from a zero guess: v0 = M f0.
+// Sweeps are out of place; on each level the iterate alternates between the
+// level's two buffers so that the last post-smoothing sweep lands in the
+// final buffer (v0 on the fine level, D.v on coarse levels).
 template <typename T>
 void vcycle(ngap_t *h, const T *f0, T *v0) {
     const int nl = h->nlev;
     hipStream_t s = h->stream;
+    const int npre = h->opts.nu_pre, npost = h->opts.nu_post;
+    std::vector<T *> pre(nl);
     for (int l = 0; l < nl - 1; l++) {
         Level &L = h->lev[l];
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
-        T *v = l == 0 ? v0 : D.v;
-        for (int sw = 0; sw < h->opts.nu_pre; sw++) {
-            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 0, sw == 0);
-            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
+        T *fin = l == 0 ? v0 : D.v;
+        T *bufs[2] = {((npre + npost) & 1) ? fin : D.w, ((npre + npost) & 1) ? D.w : fin};
+        const T *cur = nullptr;
+        int nb = 0;
+        for (int sw = 0; sw < npre; sw++) {
+            if (sw == 0)
+                launch_smooth0<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, bufs[nb], 0);
+            else
+                launch_sweep<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, bufs[nb], 0);
+            cur = bufs[nb];
+            nb ^= 1;
         }
+        pre[l] = const_cast<T *>(cur);
         Level &Cn = h->lev[l + 1];
         LevelData<T> &DC = data<T>(Cn);
-        launch_resid_restrict<T>(s, L.g, Cn.g, L.cdir, D.c, D.a0, D.a1, D.a2, f, v, DC.pm, DC.pp, DC.f);
+        // residual into the level's other buffer (free until post-smoothing), then P^T
+        T *rbuf = (cur == fin) ? D.w : fin;
+        launch_resid<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, rbuf);
+        launch_gather<T>(s, L.g, Cn.g, L.cdir, rbuf, DC.pm, DC.pp, DC.f);
     }
     {
         Level &C = h->lev[nl - 1];
@@ -653,15 +673,50 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         Level &L = h->lev[l];
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
-        T *v = l == 0 ? v0 : D.v;
+        T *fin = l == 0 ? v0 : D.v;
         Level &Cn = h->lev[l + 1];
         LevelData<T> &DC = data<T>(Cn);
-        launch_prolong<T>(s, L.g, Cn.g, L.cdir, DC.pm, DC.pp, DC.v, v);
-        for (int sw = 0; sw < h->opts.nu_post; sw++) {
-            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 1, 0);
-            launch_rbgs<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, v, 0, 0);
+        T *cur = pre[l];
+        launch_prolong_to<T>(s, L.g, Cn.g, L.cdir, DC.pm, DC.pp, DC.v, cur, cur);
+        for (int sw = 0; sw < npost; sw++) {
+            T *nxt = (cur == fin) ? D.w : fin;
+            launch_sweep<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, nxt, 1);
+            cur = nxt;
         }
     }
+}
+
+
+// ---------------------------------------------------------------- graphs
+
+void destroy_graphs(ngap_t *h) {
+    for (auto &g : h->vc_graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
+// Record the V-cycle on the handle's stream (f0 = r / r32, v0 = z / z32).
+int capture_vcycle(ngap_t *h, int prec) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    if (prec)
+        vcycle<float>(h, h->r32, h->z32);
+    else
+        vcycle<double>(h, h->r, h->z);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) {
+        g_last_hip = e;
+        return NGAP_ERR_HIP;
+    }
+    e = hipGraphInstantiate(&h->vc_graph[prec], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        g_last_hip = e;
+        h->vc_graph[prec] = nullptr;
+        return NGAP_ERR_HIP;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------------------- setup
@@ -687,6 +742,12 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     h->rcvg = rcvg;
     h->maxlevel = maxlevel;
     HIPCHK(hipSetDevice(h->device));
+    // tile kernels index with int32 and write one partial per SpMV block
+    if (h->g0.N >= (1LL << 31)) return NGAP_ERR_UNSUPPORTED;
+    {
+        int kc;
+        if (apply_blocks(h->g0, kc) > MAX_PARTS) return NGAP_ERR_UNSUPPORTED;
+    }
     int rc = ensure_base(h);
     if (rc) return rc;
     hipStream_t s = h->stream;
@@ -790,6 +851,16 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
+    {
+        int kc;
+        h->apply_parts = apply_blocks(h->lev[0].g, kc);
+    }
+    // the V-cycle launch sequence is fixed once the hierarchy exists: replay it as a graph
+    destroy_graphs(h);
+    if (op.graphs) {
+        if ((rc = capture_vcycle(h, 0))) return rc;
+        if (op.precision == NGAP_MIXED && (rc = capture_vcycle(h, 1))) return rc;
+    }
     h->setup_done = true;
     h->st.nlevels = nlev;
     h->st.coarse_singular = h->coarse_singular;
@@ -810,6 +881,10 @@ int exact_residual(ngap_t *h, double *out, int slot) {
 }
 
 void precond(ngap_t *h, bool mixed) {
+    if (h->vc_graph[mixed ? 1 : 0]) {
+        (void)hipGraphLaunch(h->vc_graph[mixed ? 1 : 0], h->stream);
+        return;
+    }
     if (mixed)
         vcycle<float>(h, h->r32, h->z32);
     else
@@ -895,7 +970,8 @@ int run_pcg(ngap_t *h) {
         }
         Level &L0 = h->lev[0];
         launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->p, h->q, h->parts);
-        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, h->flags);
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, h->apply_parts, h->scal,
+                           h->flags);
         hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
                            (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
                            h->parts + MAX_PARTS);
@@ -1091,15 +1167,16 @@ static int t_rbgs(ngap_t *h, int l, const double *f, double *v, int order) {
     Level &L = h->lev[l];
     LevelData<T> &D = data<T>(L);
     size_t n = (size_t)L.g.N;
-    T *df, *dv;
+    T *df, *dv, *dout = tmp.get<T>(n);
     int rc = up<T>(h->stream, tmp, f, n, &df);
     if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
-    if (rc) return rc;
-    int c1 = order ? 1 : 0;
-    launch_rbgs<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, c1, 0);
-    launch_rbgs<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, 1 - c1, 0);
+    if (rc || !dout) return rc ? rc : NGAP_ERR_ALLOC;
+    if (order & 2)  // first sweep from a zero iterate (the V-cycle's pre-smoothing)
+        launch_smooth0<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dout, order & 1);
+    else
+        launch_sweep<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dout, order & 1);
     HIPCHK(hipGetLastError());
-    return down<T>(h->stream, tmp, dv, n, v);
+    return down<T>(h->stream, tmp, dout, n, v);
 }
 
 template <typename T>
@@ -1110,11 +1187,12 @@ static int t_rr(ngap_t *h, int l, const double *f, const double *v, double *fc) 
     LevelData<T> &D = data<T>(L);
     LevelData<T> &DC = data<T>(C);
     size_t n = (size_t)L.g.N, nc = (size_t)C.g.N;
-    T *df, *dv, *dfc = tmp.get<T>(nc);
+    T *df, *dv, *dr = tmp.get<T>(n), *dfc = tmp.get<T>(nc);
     int rc = up<T>(h->stream, tmp, f, n, &df);
     if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
-    if (rc || !dfc) return rc ? rc : NGAP_ERR_ALLOC;
-    launch_resid_restrict<T>(h->stream, L.g, C.g, L.cdir, D.c, D.a0, D.a1, D.a2, df, dv, DC.pm, DC.pp, dfc);
+    if (rc || !dfc || !dr) return rc ? rc : NGAP_ERR_ALLOC;
+    launch_resid<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dr);
+    launch_gather<T>(h->stream, L.g, C.g, L.cdir, dr, DC.pm, DC.pp, dfc);
     HIPCHK(hipGetLastError());
     return down<T>(h->stream, tmp, dfc, nc, fc);
 }
@@ -1130,7 +1208,7 @@ static int t_pr(ngap_t *h, int l, const double *vc, double *v) {
     int rc = up<T>(h->stream, tmp, vc, nc, &dvc);
     if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
     if (rc) return rc;
-    launch_prolong<T>(h->stream, L.g, C.g, L.cdir, DC.pm, DC.pp, dvc, dv);
+    launch_prolong_to<T>(h->stream, L.g, C.g, L.cdir, DC.pm, DC.pp, dvc, dv, dv);
     HIPCHK(hipGetLastError());
     return down<T>(h->stream, tmp, dv, n, v);
 }
@@ -1174,6 +1252,7 @@ int ngap_default_opts(ngap_opts *o) {
     o->coarsest_max = 128;
     o->check_finite = 1;
     o->verbose = 0;
+    o->graphs = 1;
     return 0;
 }
 
@@ -1428,28 +1507,29 @@ int ngapt_time_kernel(ngap_t *h, int kid, int prec, int reps, double *avg_ms, do
                 return 0;
             case NGAPT_K_RBGS:
                 if (prec)
-                    launch_rbgs<float>(s, L0.g, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32, h->z32, 0, 0);
+                    launch_sweep<float>(s, L0.g, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32, h->z32, L0.s.w, 0);
                 else
-                    launch_rbgs<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r, h->z, 0, 0);
+                    launch_sweep<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r, h->z, L0.d.w, 0);
                 return 0;
             case NGAPT_K_RESTRICT: {
                 if (h->nlev < 2) return NGAP_ERR_ARG;
                 Level &C = h->lev[1];
-                if (prec)
-                    launch_resid_restrict<float>(s, L0.g, C.g, L0.cdir, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32,
-                                                 h->z32, C.s.pm, C.s.pp, C.s.f);
-                else
-                    launch_resid_restrict<double>(s, L0.g, C.g, L0.cdir, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r,
-                                                  h->z, C.d.pm, C.d.pp, C.d.f);
+                if (prec) {
+                    launch_resid<float>(s, L0.g, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32, h->z32, L0.s.w);
+                    launch_gather<float>(s, L0.g, C.g, L0.cdir, L0.s.w, C.s.pm, C.s.pp, C.s.f);
+                } else {
+                    launch_resid<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r, h->z, L0.d.w);
+                    launch_gather<double>(s, L0.g, C.g, L0.cdir, L0.d.w, C.d.pm, C.d.pp, C.d.f);
+                }
                 return 0;
             }
             case NGAPT_K_PROLONG: {
                 if (h->nlev < 2) return NGAP_ERR_ARG;
                 Level &C = h->lev[1];
                 if (prec)
-                    launch_prolong<float>(s, L0.g, C.g, L0.cdir, C.s.pm, C.s.pp, C.s.v, h->z32);
+                    launch_prolong_to<float>(s, L0.g, C.g, L0.cdir, C.s.pm, C.s.pp, C.s.v, h->z32, h->z32);
                 else
-                    launch_prolong<double>(s, L0.g, C.g, L0.cdir, C.d.pm, C.d.pp, C.d.v, h->z);
+                    launch_prolong_to<double>(s, L0.g, C.g, L0.cdir, C.d.pm, C.d.pp, C.d.v, h->z, h->z);
                 return 0;
             }
             case NGAPT_K_UPDATE: {
@@ -1468,8 +1548,8 @@ int ngapt_time_kernel(ngap_t *h, int kid, int prec, int reps, double *avg_ms, do
     double b;
     switch (kid) {
         case NGAPT_K_SPMV: b = 6.0 * w * N; break;
-        case NGAPT_K_RBGS: b = 3.5 * w * N; break;  // one colour: half of (4 coef + f + v in + v out)
-        case NGAPT_K_RESTRICT: b = (h->nlev > 1 ? (6.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
+        case NGAPT_K_RBGS: b = 7.0 * w * N; break;  // full sweep: 4 coef + f + v in + v out
+        case NGAPT_K_RESTRICT: b = (h->nlev > 1 ? (8.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
         case NGAPT_K_PROLONG: b = (h->nlev > 1 ? (2.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
         case NGAPT_K_UPDATE: b = 6.0 * 8.0 * N; break;
         case NGAPT_K_EXACT: b = (7.0 + 3.0) * 8.0 * N; break;

@@ -14,16 +14,28 @@ int grid_for(long long n);
 template <typename T>
 void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *x, T *y,
                   double *part);
+// one symmetric-order Gauss-Seidel sweep, out of place (vin == nullptr: zero
+// initial iterate); c1 = colour updated first (0 red, 1 black)
 template <typename T>
-void launch_rbgs(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f, T *v,
-                 int col, int zero_guess);
+void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                  const T *vin, T *vout, int c1);
+// first symmetric sweep from a zero iterate as a stencil (no LDS)
 template <typename T>
-void launch_resid_restrict(hipStream_t s, const Geo &F, const Geo &C, int d, const T *c, const T *a0, const T *a1,
-                           const T *a2, const T *f, const T *v, const T *pm, const T *pp, T *fc);
+void launch_smooth0(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f, T *v,
+                    int c1);
 template <typename T>
-void launch_prolong(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc, T *v);
+void launch_resid(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                  const T *x, T *r);
+// fc = P^T r
+template <typename T>
+void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc);
+// vout = vin + P vc (vout may be vin)
+template <typename T>
+void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
+                       const T *vin, T *vout);
 template <typename Tin, typename Tout>
 void launch_dense(hipStream_t s, int n, const double *Ainv, const Tin *f, Tout *v);
+int apply_blocks(const Geo &g, int &kc);
 void launch_coarse_build(hipStream_t s, const Geo &F, const Geo &C, int d, const double *c, const double *a0,
                          const double *a1, const double *a2, double *pm, double *pp, double *cc, double *ca0,
                          double *ca1, double *ca2, double *sc);
@@ -37,7 +49,8 @@ template <typename T>
 struct LevelData {
     T *c = nullptr, *a0 = nullptr, *a1 = nullptr, *a2 = nullptr;
     T *pm = nullptr, *pp = nullptr; // weights of the transfer from the finer level (coarse-indexed)
-    T *f = nullptr, *v = nullptr;   // work vectors (levels >= 1)
+    T *f = nullptr, *v = nullptr;   // right-hand side and final iterate (levels >= 1)
+    T *w = nullptr;                 // second iterate buffer (out-of-place sweeps), all levels
 };
 
 struct Level {

@@ -12,7 +12,7 @@
 namespace ngap {
 
 constexpr int NT = 256;        // threads per block of streaming kernels
-constexpr int MAX_PARTS = 1024; // partial sums per reduction (fixed order)
+constexpr int MAX_PARTS = 8192; // partial sums per reduction (fixed order)
 
 struct Geo {
     int n0, n1, n2;

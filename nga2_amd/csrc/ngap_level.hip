@@ -1,4 +1,5 @@
-// Level kernels of the PFMG-style V-cycle and the hierarchy build (gfx950).
+// Hierarchy build of the PFMG-style multigrid (gfx950): coarsening strength,
+// interpolation weights and 7-point coarse operators, FP64 -> FP32 copies.
 //
 // The algorithm (coarsening rule, interpolation weights, 7-point coarse
 // operator, red/black order) is the one stated in oracle/pcg_pfmg.c; the
@@ -23,174 +24,6 @@ __device__ __forceinline__ long long lin(const Geo &g, int i, int j, int k) {
 
 __device__ __forceinline__ int gdim(const Geo &g, int d) { return d == 0 ? g.n0 : (d == 1 ? g.n1 : g.n2); }
 __device__ __forceinline__ int gper(const Geo &g, int d) { return d == 0 ? g.p0 : (d == 1 ? g.p1 : g.p2); }
-
-// sum_nb a_nb x_nb in the order -x, +x, -y, +y, -z, +z
-template <typename T>
-__device__ __forceinline__ T nb_sum_add(const Geo &g, int i, int j, int k, long long p, const T *__restrict__ a0,
-                                        const T *__restrict__ a1, const T *__restrict__ a2, const T *x,
-                                        T s) {
-    const long long sy = g.n0, sz = (long long)g.n0 * g.n1;
-    int im = nbm(i, g.n0, g.p0), ip = nbp(i, g.n0, g.p0);
-    int jm = nbm(j, g.n1, g.p1), jp = nbp(j, g.n1, g.p1);
-    int km = nbm(k, g.n2, g.p2), kp = nbp(k, g.n2, g.p2);
-    const long long row = p - i;
-    if (im >= 0) s += a0[p] * x[row + im];
-    if (ip >= 0) { long long q = row + ip; s += a0[q] * x[q]; }
-    const long long col = p - (long long)j * sy;
-    if (jm >= 0) s += a1[p] * x[col + (long long)jm * sy];
-    if (jp >= 0) { long long q = col + (long long)jp * sy; s += a1[q] * x[q]; }
-    const long long pil = p - (long long)k * sz;
-    if (km >= 0) s += a2[p] * x[pil + (long long)km * sz];
-    if (kp >= 0) { long long q = pil + (long long)kp * sz; s += a2[q] * x[q]; }
-    return s;
-}
-
-template <typename T>
-__device__ __forceinline__ T nb_sum_sub(const Geo &g, int i, int j, int k, long long p, const T *__restrict__ a0,
-                                        const T *__restrict__ a1, const T *__restrict__ a2, const T *x,
-                                        T s) {
-    const long long sy = g.n0, sz = (long long)g.n0 * g.n1;
-    int im = nbm(i, g.n0, g.p0), ip = nbp(i, g.n0, g.p0);
-    int jm = nbm(j, g.n1, g.p1), jp = nbp(j, g.n1, g.p1);
-    int km = nbm(k, g.n2, g.p2), kp = nbp(k, g.n2, g.p2);
-    const long long row = p - i;
-    if (im >= 0) s -= a0[p] * x[row + im];
-    if (ip >= 0) { long long q = row + ip; s -= a0[q] * x[q]; }
-    const long long col = p - (long long)j * sy;
-    if (jm >= 0) s -= a1[p] * x[col + (long long)jm * sy];
-    if (jp >= 0) { long long q = col + (long long)jp * sy; s -= a1[q] * x[q]; }
-    const long long pil = p - (long long)k * sz;
-    if (km >= 0) s -= a2[p] * x[pil + (long long)km * sz];
-    if (kp >= 0) { long long q = pil + (long long)kp * sz; s -= a2[q] * x[q]; }
-    return s;
-}
-
-// ---------------------------------------------------------------- apply / smooth
-
-// y = A x on one level; when part != nullptr also writes the block's partial
-// of <x, y> (FP64) to part[blockIdx.x]. Grid-stride, fixed grid => fixed order.
-template <typename T>
-__global__ void __launch_bounds__(NT) k_apply(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
-                                              const T *__restrict__ a1, const T *__restrict__ a2,
-                                              const T *__restrict__ x, T *__restrict__ y, double *__restrict__ part) {
-    __shared__ double sm[NT / 64];
-    double acc = 0.0;
-    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
-        int i, j, k;
-        decomp(g, p, i, j, k);
-        T s = c[p] * x[p];
-        s = nb_sum_add(g, i, j, k, p, a0, a1, a2, x, s);
-        y[p] = s;
-        acc += (double)x[p] * (double)s;
-    }
-    if (part) {
-        double b = block_sum(acc, sm);
-        if (threadIdx.x == 0) part[blockIdx.x] = b;
-    }
-}
-
-// One colour of red/black Gauss-Seidel: cells with (i+j+k)&1 == col.
-// zero_guess: the other colour is taken as 0 (first sweep from v = 0).
-template <typename T>
-__global__ void __launch_bounds__(NT) k_rbgs(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
-                                             const T *__restrict__ a1, const T *__restrict__ a2,
-                                             const T *__restrict__ f, T *v, int col, int zero_guess) {
-    const int h = (g.n0 + 1) >> 1;
-    const long long M = (long long)h * g.n1 * g.n2;
-    for (long long q = blockIdx.x * (long long)NT + threadIdx.x; q < M; q += (long long)gridDim.x * NT) {
-        int ih = (int)(q % h);
-        long long r = q / h;
-        int j = (int)(r % g.n1), k = (int)(r / g.n1);
-        int i = 2 * ih + ((col + j + k) & 1);
-        if (i >= g.n0) continue;
-        long long p = (long long)i + (long long)g.n0 * ((long long)j + (long long)g.n1 * k);
-        T s = f[p];
-        if (!zero_guess) s = nb_sum_sub(g, i, j, k, p, a0, a1, a2, (const T *)v, s);
-        v[p] = s / c[p];
-    }
-}
-
-// fine residual at one cell: f - A v
-template <typename T>
-__device__ __forceinline__ T resid_at(const Geo &g, int i, int j, int k, const T *__restrict__ c,
-                                      const T *__restrict__ a0, const T *__restrict__ a1, const T *__restrict__ a2,
-                                      const T *__restrict__ f, const T *__restrict__ v) {
-    long long p = lin(g, i, j, k);
-    T t = c[p] * v[p];
-    t = nb_sum_add(g, i, j, k, p, a0, a1, a2, v, t);
-    return f[p] - t;
-}
-
-// fc = R (f - A v): one thread per coarse cell; R = P^T.
-template <typename T>
-__global__ void __launch_bounds__(NT) k_resid_restrict(Geo F, Geo C, int d, const T *__restrict__ c,
-                                                       const T *__restrict__ a0, const T *__restrict__ a1,
-                                                       const T *__restrict__ a2, const T *__restrict__ f,
-                                                       const T *__restrict__ v, const T *__restrict__ pm,
-                                                       const T *__restrict__ pp, T *__restrict__ fc) {
-    for (long long pc = blockIdx.x * (long long)NT + threadIdx.x; pc < C.N; pc += (long long)gridDim.x * NT) {
-        int J[3];
-        decomp(C, pc, J[0], J[1], J[2]);
-        int fe[3] = {J[0], J[1], J[2]};
-        fe[d] = 2 * J[d];
-        const int nd = gdim(F, d), per = gper(F, d);
-        int l = nbm(fe[d], nd, per);
-        int rr = (fe[d] + 1 < nd) ? fe[d] + 1 : -1;
-        T s = resid_at(F, fe[0], fe[1], fe[2], c, a0, a1, a2, f, v);
-        if (rr >= 0) {
-            int o[3] = {fe[0], fe[1], fe[2]};
-            o[d] = rr;
-            s += pm[pc] * resid_at(F, o[0], o[1], o[2], c, a0, a1, a2, f, v);
-        }
-        if (l >= 0) {
-            int o[3] = {fe[0], fe[1], fe[2]};
-            o[d] = l;
-            int Jl[3] = {J[0], J[1], J[2]};
-            Jl[d] = nbm(J[d], gdim(C, d), gper(C, d));
-            s += pp[lin(C, Jl[0], Jl[1], Jl[2])] * resid_at(F, o[0], o[1], o[2], c, a0, a1, a2, f, v);
-        }
-        fc[pc] = s;
-    }
-}
-
-// v += P vc: one thread per fine cell.
-template <typename T>
-__global__ void __launch_bounds__(NT) k_prolong_add(Geo F, Geo C, int d, const T *__restrict__ pm,
-                                                    const T *__restrict__ pp, const T *__restrict__ vc,
-                                                    T *__restrict__ v) {
-    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < F.N; p += (long long)gridDim.x * NT) {
-        int I[3];
-        decomp(F, p, I[0], I[1], I[2]);
-        int J[3] = {I[0], I[1], I[2]};
-        const int fd = I[d];
-        if ((fd & 1) == 0) {
-            J[d] = fd >> 1;
-            v[p] += vc[lin(C, J[0], J[1], J[2])];
-        } else {
-            J[d] = (fd - 1) >> 1;
-            long long pl = lin(C, J[0], J[1], J[2]);
-            T s = pm[pl] * vc[pl];
-            int jr = nbp(J[d], gdim(C, d), gper(C, d));
-            if (jr >= 0) {
-                int Jr[3] = {J[0], J[1], J[2]};
-                Jr[d] = jr;
-                s += pp[pl] * vc[lin(C, Jr[0], Jr[1], Jr[2])];
-            }
-            v[p] += s;
-        }
-    }
-}
-
-// v = Ainv f on the coarsest level (Ainv symmetric, FP64). One block per 256 rows.
-template <typename Tin, typename Tout>
-__global__ void __launch_bounds__(NT) k_dense_mv(int n, const double *__restrict__ Ainv, const Tin *__restrict__ f,
-                                                 Tout *__restrict__ v) {
-    int i = blockIdx.x * NT + threadIdx.x;
-    if (i >= n) return;
-    double s = 0.0;
-    for (int j = 0; j < n; j++) s += Ainv[(long long)j * n + i] * (double)f[j];
-    v[i] = (Tout)s;
-}
 
 // ---------------------------------------------------------------- hierarchy build (FP64)
 
@@ -349,47 +182,13 @@ __global__ void __launch_bounds__(NT) k_to_f64(long long n, const float *__restr
 
 // ---------------------------------------------------------------- host launchers
 
+// grid of the grid-stride streaming kernels: at most 1024 blocks, so the
+// single-block finalize of their partial sums stays short
 int grid_for(long long n) {
     long long b = (n + NT - 1) / NT;
     if (b < 1) b = 1;
-    if (b > MAX_PARTS) b = MAX_PARTS;
+    if (b > 1024) b = 1024;
     return (int)b;
-}
-
-template <typename T>
-void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *x, T *y,
-                  double *part) {
-    hipLaunchKernelGGL(k_apply<T>, dim3(grid_for(g.N)), dim3(NT), 0, s, g, c, a0, a1, a2, x, y, part);
-}
-
-template <typename T>
-void launch_rbgs(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f, T *v,
-                 int col, int zero_guess) {
-    long long M = (long long)((g.n0 + 1) / 2) * g.n1 * g.n2;
-    long long b = (M + NT - 1) / NT;
-    if (b > 65535) b = 65535;
-    hipLaunchKernelGGL(k_rbgs<T>, dim3((unsigned)b), dim3(NT), 0, s, g, c, a0, a1, a2, f, v, col, zero_guess);
-}
-
-template <typename T>
-void launch_resid_restrict(hipStream_t s, const Geo &F, const Geo &C, int d, const T *c, const T *a0, const T *a1,
-                           const T *a2, const T *f, const T *v, const T *pm, const T *pp, T *fc) {
-    long long b = (C.N + NT - 1) / NT;
-    if (b > 65535) b = 65535;
-    hipLaunchKernelGGL(k_resid_restrict<T>, dim3((unsigned)b), dim3(NT), 0, s, F, C, d, c, a0, a1, a2, f, v, pm, pp,
-                       fc);
-}
-
-template <typename T>
-void launch_prolong(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc, T *v) {
-    long long b = (F.N + NT - 1) / NT;
-    if (b > 65535) b = 65535;
-    hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)b), dim3(NT), 0, s, F, C, d, pm, pp, vc, v);
-}
-
-template <typename Tin, typename Tout>
-void launch_dense(hipStream_t s, int n, const double *Ainv, const Tin *f, Tout *v) {
-    hipLaunchKernelGGL((k_dense_mv<Tin, Tout>), dim3((n + NT - 1) / NT), dim3(NT), 0, s, n, Ainv, f, v);
 }
 
 void launch_coarse_build(hipStream_t s, const Geo &F, const Geo &C, int d, const double *c, const double *a0,
@@ -419,26 +218,5 @@ void launch_to_f32(hipStream_t s, long long n, const double *src, float *dst) {
 void launch_to_f64(hipStream_t s, long long n, const float *src, double *dst) {
     hipLaunchKernelGGL(k_to_f64, dim3(grid_for(n)), dim3(NT), 0, s, n, src, dst);
 }
-
-template void launch_apply<double>(hipStream_t, const Geo &, const double *, const double *, const double *,
-                                   const double *, const double *, double *, double *);
-template void launch_apply<float>(hipStream_t, const Geo &, const float *, const float *, const float *,
-                                  const float *, const float *, float *, double *);
-template void launch_rbgs<double>(hipStream_t, const Geo &, const double *, const double *, const double *,
-                                  const double *, const double *, double *, int, int);
-template void launch_rbgs<float>(hipStream_t, const Geo &, const float *, const float *, const float *, const float *,
-                                 const float *, float *, int, int);
-template void launch_resid_restrict<double>(hipStream_t, const Geo &, const Geo &, int, const double *,
-                                            const double *, const double *, const double *, const double *,
-                                            const double *, const double *, const double *, double *);
-template void launch_resid_restrict<float>(hipStream_t, const Geo &, const Geo &, int, const float *, const float *,
-                                           const float *, const float *, const float *, const float *, const float *,
-                                           const float *, float *);
-template void launch_prolong<double>(hipStream_t, const Geo &, const Geo &, int, const double *, const double *,
-                                     const double *, double *);
-template void launch_prolong<float>(hipStream_t, const Geo &, const Geo &, int, const float *, const float *,
-                                    const float *, float *);
-template void launch_dense<double, double>(hipStream_t, int, const double *, const double *, double *);
-template void launch_dense<float, float>(hipStream_t, int, const double *, const float *, float *);
 
 }  // namespace ngap

@@ -1,0 +1,551 @@
+// Tiled level kernels (gfx950): SpMV, fused red/black Gauss-Seidel sweep,
+// residual + restriction, prolongation + correction.
+//
+// Geometry: a level is n0 x n1 x n2 cells, p = i + n0*(j + n1*k) (int32 is
+// enough for every level this solver allocates: N < 2^31 is checked at setup).
+// Blocks tile the x-y plane; the SpMV and the sweep march through a chunk of
+// z planes so each array element is read once and z neighbours come from
+// registers.
+#include "ngap_internal.h"
+
+namespace ngap {
+
+namespace {
+
+// coordinate wrapped into [0,n) for periodic directions; -1 outside a
+// non-periodic box
+__device__ __forceinline__ int wrapc(int i, int n, int per) {
+    if (i >= 0 && i < n) return i;
+    if (!per) return -1;
+    i %= n;
+    return i < 0 ? i + n : i;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ SpMV
+// y = A x with the plane march; part (optional) gets one FP64 partial of
+// <x, y> per block. Block (TX, TY) tiles x-y; blockIdx.z selects a z chunk.
+template <typename T>
+__global__ void __launch_bounds__(256) k_apply_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
+                                                 const T *__restrict__ a1, const T *__restrict__ a2,
+                                                 const T *__restrict__ x, T *__restrict__ y,
+                                                 double *__restrict__ part, int kc) {
+    __shared__ double sm[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int plane = n0 * n1;
+    double acc = 0.0;
+    const int k0 = blockIdx.z * kc;
+    const int k1 = min(n2, k0 + kc);
+    if (i < n0 && j < n1 && k0 < k1) {
+        const int im = nbm(i, n0, g.p0), ip = nbp(i, n0, g.p0);
+        const int jm = nbm(j, n1, g.p1), jp = nbp(j, n1, g.p1);
+        const int row = j * n0;
+        const int om = im - i, op = ip - i;            // x offsets
+        const int ym = (jm - j) * n0, yp = (jp - j) * n0;  // y offsets
+        int km = nbm(k0, n2, g.p2);
+        const int col = row + i;
+        T xm = km >= 0 ? x[km * plane + col] : T(0);
+        T xc = x[k0 * plane + col];
+        T a2c = a2[k0 * plane + col];
+        for (int k = k0; k < k1; k++) {
+            const int p = k * plane + col;
+            const int kp = nbp(k, n2, g.p2);
+            const T xp = kp >= 0 ? x[kp * plane + col] : T(0);
+            const T a2p = kp >= 0 ? a2[kp * plane + col] : T(0);
+            T s = c[p] * xc;
+            if (im >= 0) s += a0[p] * x[p + om];
+            if (ip >= 0) s += a0[p + op] * x[p + op];
+            if (jm >= 0) s += a1[p] * x[p + ym];
+            if (jp >= 0) s += a1[p + yp] * x[p + yp];
+            if (km >= 0) s += a2c * xm;
+            if (kp >= 0) s += a2p * xp;
+            y[p] = s;
+            acc += (double)xc * (double)s;
+            xm = xc;
+            xc = xp;
+            a2c = a2p;
+            km = k;
+        }
+    }
+    if (part) {
+        const int t = threadIdx.y * blockDim.x + threadIdx.x;
+        const int nt = blockDim.x * blockDim.y;
+        double v = wave_sum(acc);
+        if ((t & 63) == 0) sm[t >> 6] = v;
+        __syncthreads();
+        if (t == 0) {
+            double s = 0.0;
+            for (int q = 0; q < (nt + 63) / 64; q++) s += sm[q];
+            part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------------------ stencils from zero
+// First symmetric sweep from a zero iterate, as a plain stencil:
+//   first colour c1:  v = f / c
+//   second colour:    v = (f - sum_nb a_nb f_nb / c_nb) / c
+// (the second colour reads the first colour's new values, which are f/c).
+template <typename T>
+__global__ void __launch_bounds__(256) k_smooth0_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
+                                                   const T *__restrict__ a1, const T *__restrict__ a2,
+                                                   const T *__restrict__ f, T *__restrict__ v, int c1, int kc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int plane = n0 * n1;
+    const int k0 = blockIdx.z * kc;
+    const int k1 = min(n2, k0 + kc);
+    if (i >= n0 || j >= n1 || k0 >= k1) return;
+    const int im = nbm(i, n0, g.p0), ip = nbp(i, n0, g.p0);
+    const int jm = nbm(j, n1, g.p1), jp = nbp(j, n1, g.p1);
+    const int om = im - i, op = ip - i;
+    const int ym = (jm - j) * n0, yp = (jp - j) * n0;
+    const int col = j * n0 + i;
+    int km = nbm(k0, n2, g.p2);
+    // f/c of the own column at k-1, k, k+1 and a2 at k
+    T um = km >= 0 ? f[km * plane + col] / c[km * plane + col] : T(0);
+    T cc = c[k0 * plane + col];
+    T fc = f[k0 * plane + col];
+    T a2c = a2[k0 * plane + col];
+    for (int k = k0; k < k1; k++) {
+        const int p = k * plane + col;
+        const int kp = nbp(k, n2, g.p2);
+        T cp = T(1), fp = T(0), a2p = T(0);
+        if (kp >= 0) {
+            cp = c[kp * plane + col];
+            fp = f[kp * plane + col];
+            a2p = a2[kp * plane + col];
+        }
+        const T up = kp >= 0 ? fp / cp : T(0);
+        T out;
+        if (((i + j + k) & 1) == c1) {
+            out = fc / cc;
+        } else {
+            T sum = fc;
+            if (im >= 0) sum -= a0[p] * (f[p + om] / c[p + om]);
+            if (ip >= 0) sum -= a0[p + op] * (f[p + op] / c[p + op]);
+            if (jm >= 0) sum -= a1[p] * (f[p + ym] / c[p + ym]);
+            if (jp >= 0) sum -= a1[p + yp] * (f[p + yp] / c[p + yp]);
+            if (km >= 0) sum -= a2c * um;
+            if (kp >= 0) sum -= a2p * up;
+            out = sum / cc;
+        }
+        v[p] = out;
+        um = fc / cc;
+        cc = cp;
+        fc = fp;
+        a2c = a2p;
+        km = k;
+    }
+}
+
+// r = f - A v (the residual handed to the restriction)
+template <typename T>
+__global__ void __launch_bounds__(256) k_resid_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
+                                                 const T *__restrict__ a1, const T *__restrict__ a2,
+                                                 const T *__restrict__ f, const T *__restrict__ x,
+                                                 T *__restrict__ r, int kc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int plane = n0 * n1;
+    const int k0 = blockIdx.z * kc;
+    const int k1 = min(n2, k0 + kc);
+    if (i >= n0 || j >= n1 || k0 >= k1) return;
+    const int im = nbm(i, n0, g.p0), ip = nbp(i, n0, g.p0);
+    const int jm = nbm(j, n1, g.p1), jp = nbp(j, n1, g.p1);
+    const int om = im - i, op = ip - i;
+    const int ym = (jm - j) * n0, yp = (jp - j) * n0;
+    const int col = j * n0 + i;
+    int km = nbm(k0, n2, g.p2);
+    T xm = km >= 0 ? x[km * plane + col] : T(0);
+    T xc = x[k0 * plane + col];
+    T a2c = a2[k0 * plane + col];
+    for (int k = k0; k < k1; k++) {
+        const int p = k * plane + col;
+        const int kp = nbp(k, n2, g.p2);
+        const T xp = kp >= 0 ? x[kp * plane + col] : T(0);
+        const T a2p = kp >= 0 ? a2[kp * plane + col] : T(0);
+        T t = c[p] * xc;
+        if (im >= 0) t += a0[p] * x[p + om];
+        if (ip >= 0) t += a0[p + op] * x[p + op];
+        if (jm >= 0) t += a1[p] * x[p + ym];
+        if (jp >= 0) t += a1[p + yp] * x[p + yp];
+        if (km >= 0) t += a2c * xm;
+        if (kp >= 0) t += a2p * xp;
+        r[p] = f[p] - t;
+        xm = xc;
+        xc = xp;
+        a2c = a2p;
+        km = k;
+    }
+}
+
+// fc = P^T r : one thread per coarse cell
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restrict__ r, const T *__restrict__ pm,
+                                                  const T *__restrict__ pp, T *__restrict__ fc) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= C.n0 || J >= C.n1) return;
+    const int pc = (K * C.n1 + J) * C.n0 + I;
+    const int i = D == 0 ? 2 * I : I, j = D == 1 ? 2 * J : J, k = D == 2 ? 2 * K : K;
+    const int e = D == 0 ? i : (D == 1 ? j : k);
+    const int nd = D == 0 ? F.n0 : (D == 1 ? F.n1 : F.n2);
+    const int per = D == 0 ? F.p0 : (D == 1 ? F.p1 : F.p2);
+    const int stride = D == 0 ? 1 : (D == 1 ? F.n0 : F.n0 * F.n1);
+    const int pf = (k * F.n1 + j) * F.n0 + i;
+    const int l = nbm(e, nd, per);
+    T s = r[pf];
+    if (e + 1 < nd) s += pm[pc] * r[pf + stride];
+    if (l >= 0) {
+        const int Jd = D == 0 ? I : (D == 1 ? J : K);
+        const int nc = D == 0 ? C.n0 : (D == 1 ? C.n1 : C.n2);
+        const int cper = D == 0 ? C.p0 : (D == 1 ? C.p1 : C.p2);
+        const int jl = nbm(Jd, nc, cper);
+        const int pl = D == 0 ? (K * C.n1 + J) * C.n0 + jl : (D == 1 ? (K * C.n1 + jl) * C.n0 + I
+                                                                      : (jl * C.n1 + J) * C.n0 + I);
+        s += pp[pl] * r[pf + (l - e) * stride];
+    }
+    fc[pc] = s;
+}
+
+// ------------------------------------------------------------------ fused sweep
+// One symmetric-order Gauss-Seidel sweep, out of place:
+//   first colour c1:  v1 = (f - sum a vin) / c        (vin: previous iterate)
+//   second colour:    v2 = (f - sum a v1) / c
+//   vout = v1 on c1 cells, v2 on the others.
+// Identical to the in-place two-pass red/black sweep, since each colour reads
+// only the other colour. ZERO_IN: vin == 0 (first sweep of the V-cycle).
+// A block covers (SX+2) x (SY+2) cell positions; the outer ring only computes
+// first-colour values needed by the core's second colour. The block marches
+// over planes s: at step s it computes v1 on plane s+1 and v2 on plane s.
+constexpr int SX = 64, SY = 16;
+
+// Workgroup barrier that waits only for LDS traffic, so global loads issued
+// for the next plane stay in flight across it (__syncthreads would also drain
+// them). gfx9 s_waitcnt: vmcnt = 63, expcnt = 7, lgkmcnt = 0.
+__device__ __forceinline__ void lds_barrier() {
+    __asm__ volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __builtin_amdgcn_s_barrier();
+    __asm__ volatile("" ::: "memory");
+}
+
+template <typename T, bool ZERO_IN>
+__global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
+                                                  const T *__restrict__ a1, const T *__restrict__ a2,
+                                                  const T *__restrict__ f, const T *__restrict__ vin,
+                                                  T *__restrict__ vout, int c1, int kc) {
+    __shared__ T s_vin[3][SY][SX];
+    __shared__ T s_a0[3][SY][SX];
+    __shared__ T s_a1[3][SY][SX];
+    __shared__ T s_v1[2][SY][SX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int plane = n0 * n1;
+    const int iu = blockIdx.x * (SX - 2) - 1 + tx;
+    const int ju = blockIdx.y * (SY - 2) - 1 + ty;
+    const int ii = wrapc(iu, n0, g.p0);
+    const int jj = wrapc(ju, n1, g.p1);
+    const bool valid = ii >= 0 && jj >= 0;
+    const int col = valid ? jj * n0 + ii : 0;
+    const bool left = tx == 0, right = tx == SX - 1, bot = ty == 0, top = ty == SY - 1;
+    const bool corner = (left || right) && (bot || top);
+    const bool core = !left && !right && !bot && !top && iu < n0 && ju < n1;
+    // the one cell outside the tile a ring position couples to (-1: none)
+    int cout = -1;
+    if (valid && !corner) {
+        if (left) { int q = wrapc(ii - 1, n0, g.p0); cout = q >= 0 ? jj * n0 + q : -1; }
+        if (right) { int q = wrapc(ii + 1, n0, g.p0); cout = q >= 0 ? jj * n0 + q : -1; }
+        if (bot) { int q = wrapc(jj - 1, n1, g.p1); cout = q >= 0 ? q * n0 + ii : -1; }
+        if (top) { int q = wrapc(jj + 1, n1, g.p1); cout = q >= 0 ? q * n0 + ii : -1; }
+    }
+    // coupling coefficient to that outside cell lives there for +x/+y, here for -x/-y
+    const T *__restrict__ aout = right ? a0 : a1;
+    const bool out_owns = right || top;
+    const int par = (ii + jj) & 1;  // colour of the cell on plane k is (par + k) & 1
+    const int k0 = blockIdx.z * kc;
+    const int k1 = min(n2, k0 + kc);
+
+    auto kwrap = [&](int k) { return wrapc(k, n2, g.p2); };
+    auto ld = [&](const T *__restrict__ a, int kk) -> T { return (valid && kk >= 0) ? a[kk * plane + col] : T(0); };
+    auto ldo = [&](const T *__restrict__ a, int kk) -> T { return (cout >= 0 && kk >= 0) ? a[kk * plane + cout] : T(0); };
+
+    int s = k0 - 2;
+    // data of plane s+1 (coefficients, vin, outside values) and of plane s+2 (vin, a2)
+    int kA = kwrap(s + 1);
+    T vin_s = ZERO_IN ? T(0) : ld(vin, kwrap(s));
+    T c_A = valid && kA >= 0 ? c[kA * plane + col] : T(1);
+    T f_A = ld(f, kA), a0_A = ld(a0, kA), a1_A = ld(a1, kA), a2_A = ld(a2, kA);
+    T vin_A = ZERO_IN ? T(0) : ld(vin, kA);
+    T vo_A = ZERO_IN ? T(0) : ldo(vin, kA);
+    T ao_A = out_owns ? ldo(aout, kA) : T(0);
+    int kB = kwrap(s + 2);
+    T vin_B = ZERO_IN ? T(0) : ld(vin, kB);
+    T a2_B = ld(a2, kB);
+    T c_s = T(1), f_s = T(0), a0_s = T(0), a1_s = T(0), a2_s = T(0);
+    T v1_m = T(0), v1_0 = T(0);
+    int slot3 = 0;  // slot of plane s+1 in the 3-deep ring
+    for (; s < k1; s++) {
+        // prefetch: plane s+2 coefficients and outside values, plane s+3 vin and a2
+        const int kN = kwrap(s + 2), kN2 = kwrap(s + 3);
+        const T c_N = valid && kN >= 0 ? c[kN * plane + col] : T(1);
+        const T f_N = ld(f, kN), a0_N = ld(a0, kN), a1_N = ld(a1, kN);
+        const T vo_N = ZERO_IN ? T(0) : ldo(vin, kN);
+        const T ao_N = out_owns ? ldo(aout, kN) : T(0);
+        const T vin_N2 = ZERO_IN ? T(0) : ld(vin, kN2);
+        const T a2_N2 = ld(a2, kN2);
+        // publish plane s+1
+        if (!ZERO_IN) s_vin[slot3][ty][tx] = vin_A;
+        s_a0[slot3][ty][tx] = a0_A;
+        s_a1[slot3][ty][tx] = a1_A;
+        lds_barrier();
+        const int ks = kwrap(s);
+        // first colour on plane s+1
+        T v1_p = T(0);
+        if (valid && kA >= 0 && !corner && ((par + kA) & 1) == c1) {
+            T sum = f_A;
+            if (!ZERO_IN) {
+                const T xm = left ? vo_A : s_vin[slot3][ty][tx - 1];
+                const T xp = right ? vo_A : s_vin[slot3][ty][tx + 1];
+                const T ym = bot ? vo_A : s_vin[slot3][ty - 1][tx];
+                const T yp = top ? vo_A : s_vin[slot3][ty + 1][tx];
+                const T a0p = right ? ao_A : s_a0[slot3][ty][tx + 1];
+                const T a1p = top ? ao_A : s_a1[slot3][ty + 1][tx];
+                sum -= a0_A * xm;
+                sum -= a0p * xp;
+                sum -= a1_A * ym;
+                sum -= a1p * yp;
+                sum -= a2_A * vin_s;
+                sum -= a2_B * vin_B;
+            }
+            v1_p = sum / c_A;
+        }
+        s_v1[(s + 1) & 1][ty][tx] = v1_p;
+        // second colour on plane s (core only)
+        if (s >= k0 && core && ks >= 0) {
+            T out;
+            if (((par + ks) & 1) == c1) {
+                out = v1_0;
+            } else {
+                const int sl = (slot3 + 2) % 3;  // slot of plane s
+                T sum = f_s;
+                sum -= a0_s * s_v1[s & 1][ty][tx - 1];
+                sum -= s_a0[sl][ty][tx + 1] * s_v1[s & 1][ty][tx + 1];
+                sum -= a1_s * s_v1[s & 1][ty - 1][tx];
+                sum -= s_a1[sl][ty + 1][tx] * s_v1[s & 1][ty + 1][tx];
+                sum -= a2_s * v1_m;
+                sum -= a2_A * v1_p;
+                out = sum / c_s;
+            }
+            vout[ks * plane + col] = out;
+        }
+        // rotate: plane s+1 -> s, prefetched plane s+2 -> s+1
+        vin_s = vin_A;
+        c_s = c_A;
+        f_s = f_A;
+        a0_s = a0_A;
+        a1_s = a1_A;
+        a2_s = a2_A;
+        kA = kN;
+        c_A = c_N;
+        f_A = f_N;
+        a0_A = a0_N;
+        a1_A = a1_N;
+        a2_A = a2_B;
+        vin_A = vin_B;
+        vo_A = vo_N;
+        ao_A = ao_N;
+        vin_B = vin_N2;
+        a2_B = a2_N2;
+        v1_m = v1_0;
+        v1_0 = v1_p;
+        slot3 = (slot3 + 1) % 3;
+    }
+}
+
+// ------------------------------------------------------------------ restriction
+__device__ __forceinline__ int gdim_(const Geo &g, int d) { return d == 0 ? g.n0 : (d == 1 ? g.n1 : g.n2); }
+__device__ __forceinline__ int gper_(const Geo &g, int d) { return d == 0 ? g.p0 : (d == 1 ? g.p1 : g.p2); }
+
+// vout = vin + P vc: one thread per fine cell (vout may alias vin)
+template <typename T, int D>
+__global__ void __launch_bounds__(256) k_prolong_t(Geo F, Geo C, const T *__restrict__ pm, const T *__restrict__ pp,
+                                                   const T *__restrict__ vc, const T *vin, T *vout) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= F.n0 || j >= F.n1) return;
+    const int p = (k * F.n1 + j) * F.n0 + i;
+    const int fd = D == 0 ? i : (D == 1 ? j : k);
+    const int Jd = fd >> 1;
+    const int I = D == 0 ? Jd : i, J = D == 1 ? Jd : j, K = D == 2 ? Jd : k;
+    const int pl = (K * C.n1 + J) * C.n0 + I;
+    T add;
+    if ((fd & 1) == 0) {
+        add = vc[pl];
+    } else {
+        add = pm[pl] * vc[pl];
+        const int nc = D == 0 ? C.n0 : (D == 1 ? C.n1 : C.n2);
+        const int cper = D == 0 ? C.p0 : (D == 1 ? C.p1 : C.p2);
+        const int jr = nbp(Jd, nc, cper);
+        if (jr >= 0) {
+            const int pr = D == 0 ? (K * C.n1 + J) * C.n0 + jr : (D == 1 ? (K * C.n1 + jr) * C.n0 + I
+                                                                          : (jr * C.n1 + J) * C.n0 + I);
+            add += pp[pl] * vc[pr];
+        }
+    }
+    vout[p] = vin[p] + add;
+}
+
+// coarsest level: v = Ainv f, one wavefront per row (Ainv symmetric)
+template <typename Tin, typename Tout>
+__global__ void __launch_bounds__(256) k_dense_t(int n, const double *__restrict__ Ainv, const Tin *__restrict__ f,
+                                                 Tout *__restrict__ v) {
+    const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    double s = 0.0;
+    for (int j = lane; j < n; j += 64) s += Ainv[(long long)row * n + j] * (double)f[j];
+    s = wave_sum(s);
+    if (lane == 0) v[row] = (Tout)s;
+}
+
+// ------------------------------------------------------------------ launchers
+
+namespace {
+int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+}  // namespace
+
+// x-y tile for 256-thread blocks adapted to small planes
+void tile_dims(const Geo &g, int &tx, int &ty) {
+    tx = std::min(64, pow2ceil(g.n0));
+    ty = std::min(256 / tx, pow2ceil(g.n1));
+}
+
+int apply_blocks(const Geo &g, int &kc) {
+    int tx, ty;
+    tile_dims(g, tx, ty);
+    const int gx = (g.n0 + tx - 1) / tx, gy = (g.n1 + ty - 1) / ty;
+    long long tiles = (long long)gx * gy;
+    long long want = 4096;  // 256-thread blocks; z chunks of ~16 planes at 256^3
+    kc = (int)std::max<long long>(1, (g.n2 * tiles + want - 1) / want);
+    if (kc > g.n2) kc = g.n2;
+    int gz = (g.n2 + kc - 1) / kc;
+    while ((long long)gz * tiles > MAX_PARTS && kc < g.n2) {
+        kc++;
+        gz = (g.n2 + kc - 1) / kc;
+    }
+    return (int)(tiles * gz);
+}
+
+template <typename T>
+void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *x, T *y,
+                  double *part) {
+    int tx, ty, kc;
+    tile_dims(g, tx, ty);
+    apply_blocks(g, kc);
+    dim3 grid((g.n0 + tx - 1) / tx, (g.n1 + ty - 1) / ty, (g.n2 + kc - 1) / kc);
+    hipLaunchKernelGGL(k_apply_t<T>, grid, dim3(tx, ty), 0, s, g, c, a0, a1, a2, x, y, part, kc);
+}
+
+template <typename T>
+void launch_smooth0(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f, T *v,
+                    int c1) {
+    int tx, ty, kc;
+    tile_dims(g, tx, ty);
+    apply_blocks(g, kc);
+    dim3 grid((g.n0 + tx - 1) / tx, (g.n1 + ty - 1) / ty, (g.n2 + kc - 1) / kc);
+    hipLaunchKernelGGL(k_smooth0_t<T>, grid, dim3(tx, ty), 0, s, g, c, a0, a1, a2, f, v, c1, kc);
+}
+
+template <typename T>
+void launch_resid(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                  const T *x, T *r) {
+    int tx, ty, kc;
+    tile_dims(g, tx, ty);
+    apply_blocks(g, kc);
+    dim3 grid((g.n0 + tx - 1) / tx, (g.n1 + ty - 1) / ty, (g.n2 + kc - 1) / kc);
+    hipLaunchKernelGGL(k_resid_t<T>, grid, dim3(tx, ty), 0, s, g, c, a0, a1, a2, f, x, r, kc);
+}
+
+template <typename T>
+void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc) {
+    int tx, ty;
+    tile_dims(C, tx, ty);
+    dim3 grid((C.n0 + tx - 1) / tx, (C.n1 + ty - 1) / ty, C.n2);
+    if (d == 0)
+        hipLaunchKernelGGL((k_gather_t<T, 0>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
+    else if (d == 1)
+        hipLaunchKernelGGL((k_gather_t<T, 1>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
+    else
+        hipLaunchKernelGGL((k_gather_t<T, 2>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
+}
+
+template <typename T>
+void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                  const T *vin, T *vout, int c1) {
+    const int gx = (g.n0 + SX - 3) / (SX - 2), gy = (g.n1 + SY - 3) / (SY - 2);
+    const long long tiles = (long long)gx * gy;
+    // about one round of resident 1024-thread blocks (2 per CU on 256 CUs);
+    // each z chunk recomputes 3 planes, so chunks stay >= 16 planes when possible
+    long long gz = std::max<long long>(1, 512 / tiles);
+    if (gz > 1 && (g.n2 + gz - 1) / gz < 16) gz = std::max<long long>(1, g.n2 / 16);
+    int kc = (int)((g.n2 + gz - 1) / gz);
+    if (kc < 1) kc = 1;
+    dim3 grid(gx, gy, (g.n2 + kc - 1) / kc);
+    if (vin)
+        hipLaunchKernelGGL((k_sweep<T, false>), grid, dim3(SX, SY), 0, s, g, c, a0, a1, a2, f, vin, vout, c1, kc);
+    else
+        hipLaunchKernelGGL((k_sweep<T, true>), grid, dim3(SX, SY), 0, s, g, c, a0, a1, a2, f, (const T *)nullptr,
+                           vout, c1, kc);
+}
+
+template <typename T>
+void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
+                       const T *vin, T *vout) {
+    int tx, ty;
+    tile_dims(F, tx, ty);
+    dim3 grid((F.n0 + tx - 1) / tx, (F.n1 + ty - 1) / ty, F.n2);
+    if (d == 0)
+        hipLaunchKernelGGL((k_prolong_t<T, 0>), grid, dim3(tx, ty), 0, s, F, C, pm, pp, vc, vin, vout);
+    else if (d == 1)
+        hipLaunchKernelGGL((k_prolong_t<T, 1>), grid, dim3(tx, ty), 0, s, F, C, pm, pp, vc, vin, vout);
+    else
+        hipLaunchKernelGGL((k_prolong_t<T, 2>), grid, dim3(tx, ty), 0, s, F, C, pm, pp, vc, vin, vout);
+}
+
+template <typename Tin, typename Tout>
+void launch_dense(hipStream_t s, int n, const double *Ainv, const Tin *f, Tout *v) {
+    hipLaunchKernelGGL((k_dense_t<Tin, Tout>), dim3((n + 3) / 4), dim3(256), 0, s, n, Ainv, f, v);
+}
+
+#define INST(T)                                                                                                    \
+    template void launch_apply<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
+                                  T *, double *);                                                                  \
+    template void launch_sweep<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
+                                  const T *, T *, int);                                                            \
+    template void launch_smooth0<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *,          \
+                                    const T *, T *, int);                                                          \
+    template void launch_resid<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
+                                  const T *, T *);                                                                 \
+    template void launch_gather<T>(hipStream_t, const Geo &, const Geo &, int, const T *, const T *, const T *,    \
+                                   T *);                                                                           \
+    template void launch_prolong_to<T>(hipStream_t, const Geo &, const Geo &, int, const T *, const T *,           \
+                                       const T *, const T *, T *);                                                 \
+    template void launch_dense<T, T>(hipStream_t, int, const double *, const T *, T *);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace ngap

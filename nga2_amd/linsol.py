@@ -274,12 +274,22 @@ class DeviceSolve:
                                                stc.ctypes.data_as(C.POINTER(C.c_int)), method))
         check("ngap_init", lib().ngap_init(self._h))
 
+    @staticmethod
+    def _torch_sync(t):
+        # torch runs on its own HIP runtime: finish its queued work on t's
+        # device before the library reads the buffer
+        import torch
+
+        torch.cuda.synchronize(t.device)
+
     def setup(self, d_opr, maxit=200, rcvg=1e-10, maxlevel=0, **opts):
+        self._torch_sync(d_opr)
         o = _lib.default_opts(**opts)
         check("ngap_setup_device", lib().ngap_setup_device(self._h, C.c_void_p(d_opr.data_ptr()), int(maxit),
                                                            float(rcvg), int(maxlevel), C.byref(o)))
 
     def solve(self, d_rhs, d_sol):
+        self._torch_sync(d_rhs)
         it = C.c_int()
         rerr = C.c_double()
         aerr = C.c_double()

@@ -29,7 +29,9 @@
  *  diagonal, non-periodic ones are dropped (HYPRE ignores off-grid entries).
  *
  *  Coarsening: one direction per level (semicoarsening). Candidates are the
- *  directions with n_d >= 4 and even (periodic) or n_d >= 2 (non-periodic);
+ *  directions with n_d a multiple of 4 (periodic: the coarse length stays
+ *  even, as red/black colouring needs across the wrap) or n_d >= 2
+ *  (non-periodic);
  *  the candidate with the largest sum |a_d| wins (ties within 1e-6 relative
  *  go to the lower index). Coarse point J along d is fine point 2J; fine odd
  *  points o = 2J+1 interpolate from J and J+1 with operator-dependent weights
@@ -46,7 +48,7 @@
  *  Coarsening stops at maxlevel levels, at <= coarsest_max cells or when no
  *  direction can be coarsened. The coarsest level is solved exactly by dense
  *  Cholesky (rank-one regularised when its row sums vanish, i.e. singular)
- *  when it has <= 4096 cells, otherwise by 8 symmetric RB/BR sweeps.
+ *  when it has <= 1024 cells, otherwise by 8 symmetric RB/BR sweeps.
  *
  *  Red cells have (i+j+k) even. Periodic directions of odd length >= 3 are
  *  rejected (red/black colouring is inconsistent across the wrap).
@@ -280,7 +282,7 @@ static double row_sum(const lvl_t *L, int i, int j, int k) {
 }
 
 static int can_coarsen(const lvl_t *L, int d) {
-    if (L->per[d]) return (L->n[d] % 2 == 0) && L->n[d] >= 4;
+    if (L->per[d]) return (L->n[d] % 4) == 0; /* coarse periodic length stays even */
     return L->n[d] >= 2;
 }
 
@@ -672,7 +674,7 @@ int orc_create(orc_t **out, int nx, int ny, int nz, int xper, int yper, int zper
         o->nlev++;
     }
     lvl_t *C = &o->L[o->nlev - 1];
-    o->coarse_dense = C->N <= 4096;
+    o->coarse_dense = C->N <= 1024;
     if (o->coarse_dense) {
         int rc = factor_coarsest(o);
         if (rc) { orc_destroy(o); return rc == -1 ? -1 : -4; }

@@ -15,6 +15,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libngapsolve.so on cuda:0)")
 
 
+@pytest.fixture(scope="session", autouse=True)
+def _torch_hip_first(request):
+    """torch ships its own HIP runtime; it must initialise the device before
+    libngapsolve.so (system ROCm) does, or torch reports no GPU afterwards."""
+    if any(item.get_closest_marker("gpu") for item in request.session.items):
+        import torch
+
+        if torch.cuda.is_available():
+            torch.cuda.init()
+            torch.cuda.set_device(0)
+    yield
+
+
 def closed_form_periodic(b_flat, n, h):
     """Exact solution of the uniform periodic 7-point problem -vol*lap x = b
     (vol = h^3) by FFT, zero mean (the fourier3d method,

@@ -96,6 +96,9 @@ def test_level_kernels_fp64(case):
         assert _close(hk.apply(l, x), o.apply(l, x), 1e-13), ("apply", l)
         for order in (0, 1):
             assert _close(hk.rbgs(l, f, x, order), o.rbgs(l, f, x, order), 1e-12), ("rbgs", l, order)
+            # first sweep from a zero iterate (stencil form of the pre-smoother)
+            zero = np.zeros(n)
+            assert _close(hk.rbgs(l, f, x, 2 | order), o.rbgs(l, f, zero, order), 1e-12), ("rbgs0", l, order)
         if l + 1 < o.nlevels:
             assert _close(hk.resid_restrict(l, f, x), o.resid_restrict(l, f, x), 1e-12), ("restrict", l)
             vc = rng.standard_normal(o.level_size(l + 1))
