@@ -21,6 +21,27 @@ __device__ __forceinline__ int wrapc(int i, int n, int per) {
     return i < 0 ? i + n : i;
 }
 
+// XCD-aware tile order. The dispatcher deals workgroups round-robin to the 8
+// XCDs (workgroup b runs on XCD b % 8), each with its own L2. Remap the
+// linear workgroup id so every XCD walks one contiguous range of tiles: tiles
+// adjacent in y and z then share an L2, and their halo rows are hits.
+// Bijective on [0, nb) for any nb.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+    const int q = nb >> 3, r = nb & 7;
+    const int x = b & 7, m = b >> 3;
+    return x < r ? x * (q + 1) + m : r * (q + 1) + (x - r) * q + m;
+}
+
+// tile coordinates of this workgroup for a 1-D launch of gx*gy*gz groups
+__device__ __forceinline__ void tile_of(int gx, int gy, int &bx, int &by, int &bz) {
+    const int nb = gridDim.x;
+    const int t = xcd_tile(blockIdx.x, nb);
+    bx = t % gx;
+    const int r = t / gx;
+    by = r % gy;
+    bz = r / gy;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ SpMV
@@ -30,14 +51,16 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_apply_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                  const T *__restrict__ a1, const T *__restrict__ a2,
                                                  const T *__restrict__ x, T *__restrict__ y,
-                                                 double *__restrict__ part, int kc) {
+                                                 double *__restrict__ part, int kc, int gx, int gy) {
     __shared__ double sm[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int bx, by, bz;
+    tile_of(gx, gy, bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
     const int plane = n0 * n1;
     double acc = 0.0;
-    const int k0 = blockIdx.z * kc;
+    const int k0 = bz * kc;
     const int k1 = min(n2, k0 + kc);
     if (i < n0 && j < n1 && k0 < k1) {
         const int im = nbm(i, n0, g.p0), ip = nbp(i, n0, g.p0);
@@ -79,7 +102,7 @@ __global__ void __launch_bounds__(256) k_apply_t(Geo g, const T *__restrict__ c,
         if (t == 0) {
             double s = 0.0;
             for (int q = 0; q < (nt + 63) / 64; q++) s += sm[q];
-            part[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+            part[blockIdx.x] = s;
         }
     }
 }
@@ -92,12 +115,14 @@ __global__ void __launch_bounds__(256) k_apply_t(Geo g, const T *__restrict__ c,
 template <typename T>
 __global__ void __launch_bounds__(256) k_smooth0_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                    const T *__restrict__ a1, const T *__restrict__ a2,
-                                                   const T *__restrict__ f, T *__restrict__ v, int c1, int kc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+                                                   const T *__restrict__ f, T *__restrict__ v, int c1, int kc, int gx, int gy) {
+    int bx, by, bz;
+    tile_of(gx, gy, bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
     const int plane = n0 * n1;
-    const int k0 = blockIdx.z * kc;
+    const int k0 = bz * kc;
     const int k1 = min(n2, k0 + kc);
     if (i >= n0 || j >= n1 || k0 >= k1) return;
     const int im = nbm(i, n0, g.p0), ip = nbp(i, n0, g.p0);
@@ -148,12 +173,14 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_resid_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                  const T *__restrict__ a1, const T *__restrict__ a2,
                                                  const T *__restrict__ f, const T *__restrict__ x,
-                                                 T *__restrict__ r, int kc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+                                                 T *__restrict__ r, int kc, int gx, int gy) {
+    int bx, by, bz;
+    tile_of(gx, gy, bx, by, bz);
+    const int i = bx * blockDim.x + threadIdx.x;
+    const int j = by * blockDim.y + threadIdx.y;
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
     const int plane = n0 * n1;
-    const int k0 = blockIdx.z * kc;
+    const int k0 = bz * kc;
     const int k1 = min(n2, k0 + kc);
     if (i >= n0 || j >= n1 || k0 >= k1) return;
     const int im = nbm(i, n0, g.p0), ip = nbp(i, n0, g.p0);
@@ -225,7 +252,7 @@ __global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restr
 // A block covers (SX+2) x (SY+2) cell positions; the outer ring only computes
 // first-colour values needed by the core's second colour. The block marches
 // over planes s: at step s it computes v1 on plane s+1 and v2 on plane s.
-constexpr int SX = 64, SY = 16;
+constexpr int SX = 64;
 
 // Workgroup barrier that waits only for LDS traffic, so global loads issued
 // for the next plane stay in flight across it (__syncthreads would also drain
@@ -237,11 +264,11 @@ __device__ __forceinline__ void lds_barrier() {
     __asm__ volatile("" ::: "memory");
 }
 
-template <typename T, bool ZERO_IN>
+template <typename T, bool ZERO_IN, int SY>
 __global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                   const T *__restrict__ a1, const T *__restrict__ a2,
                                                   const T *__restrict__ f, const T *__restrict__ vin,
-                                                  T *__restrict__ vout, int c1, int kc) {
+                                                  T *__restrict__ vout, int c1, int kc, int gx, int gy) {
     __shared__ T s_vin[3][SY][SX];
     __shared__ T s_a0[3][SY][SX];
     __shared__ T s_a1[3][SY][SX];
@@ -249,8 +276,10 @@ __global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
     const int plane = n0 * n1;
-    const int iu = blockIdx.x * (SX - 2) - 1 + tx;
-    const int ju = blockIdx.y * (SY - 2) - 1 + ty;
+    int bx, by, bz;
+    tile_of(gx, gy, bx, by, bz);
+    const int iu = bx * (SX - 2) - 1 + tx;
+    const int ju = by * (SY - 2) - 1 + ty;
     const int ii = wrapc(iu, n0, g.p0);
     const int jj = wrapc(ju, n1, g.p1);
     const bool valid = ii >= 0 && jj >= 0;
@@ -270,7 +299,7 @@ __global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c
     const T *__restrict__ aout = right ? a0 : a1;
     const bool out_owns = right || top;
     const int par = (ii + jj) & 1;  // colour of the cell on plane k is (par + k) & 1
-    const int k0 = blockIdx.z * kc;
+    const int k0 = bz * kc;
     const int k1 = min(n2, k0 + kc);
 
     auto kwrap = [&](int k) { return wrapc(k, n2, g.p2); };
@@ -455,8 +484,9 @@ void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
     int tx, ty, kc;
     tile_dims(g, tx, ty);
     apply_blocks(g, kc);
-    dim3 grid((g.n0 + tx - 1) / tx, (g.n1 + ty - 1) / ty, (g.n2 + kc - 1) / kc);
-    hipLaunchKernelGGL(k_apply_t<T>, grid, dim3(tx, ty), 0, s, g, c, a0, a1, a2, x, y, part, kc);
+    const int gx = (g.n0 + tx - 1) / tx, gy = (g.n1 + ty - 1) / ty, gz = (g.n2 + kc - 1) / kc;
+    hipLaunchKernelGGL(k_apply_t<T>, dim3(gx * gy * gz), dim3(tx, ty), 0, s, g, c, a0, a1, a2, x, y, part, kc, gx,
+                       gy);
 }
 
 template <typename T>
@@ -465,8 +495,9 @@ void launch_smooth0(hipStream_t s, const Geo &g, const T *c, const T *a0, const 
     int tx, ty, kc;
     tile_dims(g, tx, ty);
     apply_blocks(g, kc);
-    dim3 grid((g.n0 + tx - 1) / tx, (g.n1 + ty - 1) / ty, (g.n2 + kc - 1) / kc);
-    hipLaunchKernelGGL(k_smooth0_t<T>, grid, dim3(tx, ty), 0, s, g, c, a0, a1, a2, f, v, c1, kc);
+    const int gx = (g.n0 + tx - 1) / tx, gy = (g.n1 + ty - 1) / ty, gz = (g.n2 + kc - 1) / kc;
+    hipLaunchKernelGGL(k_smooth0_t<T>, dim3(gx * gy * gz), dim3(tx, ty), 0, s, g, c, a0, a1, a2, f, v, c1, kc, gx,
+                       gy);
 }
 
 template <typename T>
@@ -475,8 +506,9 @@ void launch_resid(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
     int tx, ty, kc;
     tile_dims(g, tx, ty);
     apply_blocks(g, kc);
-    dim3 grid((g.n0 + tx - 1) / tx, (g.n1 + ty - 1) / ty, (g.n2 + kc - 1) / kc);
-    hipLaunchKernelGGL(k_resid_t<T>, grid, dim3(tx, ty), 0, s, g, c, a0, a1, a2, f, x, r, kc);
+    const int gx = (g.n0 + tx - 1) / tx, gy = (g.n1 + ty - 1) / ty, gz = (g.n2 + kc - 1) / kc;
+    hipLaunchKernelGGL(k_resid_t<T>, dim3(gx * gy * gz), dim3(tx, ty), 0, s, g, c, a0, a1, a2, f, x, r, kc, gx,
+                       gy);
 }
 
 template <typename T>
@@ -492,23 +524,33 @@ void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r,
         hipLaunchKernelGGL((k_gather_t<T, 2>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
 }
 
-template <typename T>
-void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
-                  const T *vin, T *vout, int c1) {
+template <typename T, int SY>
+static void launch_sweep_sy(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2,
+                            const T *f, const T *vin, T *vout, int c1, long long target) {
     const int gx = (g.n0 + SX - 3) / (SX - 2), gy = (g.n1 + SY - 3) / (SY - 2);
     const long long tiles = (long long)gx * gy;
-    // about one round of resident 1024-thread blocks (2 per CU on 256 CUs);
-    // each z chunk recomputes 3 planes, so chunks stay >= 16 planes when possible
-    long long gz = std::max<long long>(1, 512 / tiles);
+    // about `target` workgroups; each z chunk recomputes 3 planes, so chunks
+    // stay >= 16 planes when possible
+    long long gz = std::max<long long>(1, target / tiles);
     if (gz > 1 && (g.n2 + gz - 1) / gz < 16) gz = std::max<long long>(1, g.n2 / 16);
     int kc = (int)((g.n2 + gz - 1) / gz);
     if (kc < 1) kc = 1;
-    dim3 grid(gx, gy, (g.n2 + kc - 1) / kc);
+    const int nz = (g.n2 + kc - 1) / kc;
+    dim3 grid((unsigned)(gx * gy * nz));
     if (vin)
-        hipLaunchKernelGGL((k_sweep<T, false>), grid, dim3(SX, SY), 0, s, g, c, a0, a1, a2, f, vin, vout, c1, kc);
+        hipLaunchKernelGGL((k_sweep<T, false, SY>), grid, dim3(SX, SY), 0, s, g, c, a0, a1, a2, f, vin, vout, c1, kc,
+                           gx, gy);
     else
-        hipLaunchKernelGGL((k_sweep<T, true>), grid, dim3(SX, SY), 0, s, g, c, a0, a1, a2, f, (const T *)nullptr,
-                           vout, c1, kc);
+        hipLaunchKernelGGL((k_sweep<T, true, SY>), grid, dim3(SX, SY), 0, s, g, c, a0, a1, a2, f,
+                           (const T *)nullptr, vout, c1, kc, gx, gy);
+}
+
+// 64 x 16 workgroups (1024 threads, two per CU); ~512 of them per launch at
+// 256^3 measured best among 64 x {4, 8, 16} tiles and 512..4096 groups
+template <typename T>
+void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                  const T *vin, T *vout, int c1) {
+    launch_sweep_sy<T, 16>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
 }
 
 template <typename T>
