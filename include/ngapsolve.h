@@ -68,6 +68,7 @@ extern "C" {
 #define NGAP_ERR_BREAKDOWN 3      /* <p,Ap> <= 0 or non-finite in the FP64 iteration */
 #define NGAP_ERR_ALLOC 4          /* device or host allocation failed */
 #define NGAP_ERR_COARSE 5         /* coarsest-level factorisation failed */
+#define NGAP_ERR_STAGE 6          /* the correction stage returned an error */
 
 /* ngap_stats.status */
 #define NGAP_STATUS_CONVERGED 0
@@ -102,9 +103,10 @@ typedef struct ngap_stats {
     int nlevels;         /* multigrid levels including the fine level */
     int coarse_singular; /* coarsest operator had vanishing row sums */
     int precision;       /* mode actually used by the last solve */
-    int fallback;        /* 1 if a mixed solve fell back to the FP64 V-cycle */
+    int fallback;        /* bit 0: a mixed solve fell back to the FP64 V-cycle;
+                            bit 1: the correction stage was dropped (guess rejected or breakdown) */
     int restarts;        /* residual replacements */
-    int pad0;
+    int stage_calls;     /* calls of the correction stage (ngap_set_stage) in the last solve */
     double rerr;         /* relative 2-norm of the recursive residual at exit */
     double true_rerr;    /* ||b - A x||_2/||b||_2 with the exact 7-array opr */
     double aerr;         /* ||b - A x||_2 */
@@ -149,6 +151,18 @@ int ngap_solve(ngap_t *h, const double *rhs, double *sol, int *it, double *rerr,
  * and d_sol are read, and the handle's stream before the call returns. */
 int ngap_solve_device(ngap_t *h, const double *d_rhs, double *d_sol, int *it, double *rerr, double *aerr);
 
+/* res = rhs - A sol with the exact operator of the last setup, for rhs and
+ * sol in device memory (ghosted layout, as ngap_solve_device) and res a
+ * device vector of the nx*ny*nz interior cells, x fastest. *rnorm (host,
+ * may be NULL) = ||res||_2. No counterpart in the reference: it produces the
+ * inputs of a correction stage (training pairs of a learned guess). */
+int ngap_residual_device(ngap_t *h, const double *d_rhs, const double *d_sol, double *d_res, double *rnorm);
+
+/* y = A x for device vectors of the nx*ny*nz interior cells (x fastest),
+ * with the fine-level FP64 operator the PCG iterates with. No counterpart in
+ * the reference: it supplies the residual-norm loss of a learned stage. */
+int ngap_apply_device(ngap_t *h, const double *d_x, double *d_y);
+
 /* Changes maxit/rcvg without rebuilding the hierarchy. */
 int ngap_set_tolerance(ngap_t *h, int maxit, double rcvg);
 
@@ -158,6 +172,23 @@ int ngap_destroy(ngap_t *h);
 /* Statistics of the last setup/solve (replaces reading it/rerr/aerr and the
  * log/print routines, :426-463). */
 int ngap_last_stats(const ngap_t *h, ngap_stats *s);
+
+/* Correction stage: fn(ctx, r, e, n, stream) writes e ~ A^{-1} r for r and e
+ * device FP64 vectors of the n interior cells (x fastest), launching its work
+ * on `stream` (the handle's hipStream_t) without synchronizing it.
+ * No counterpart in the reference (hypre_str has no learned stage). flags:
+ *   NGAP_STAGE_GUESS    x0 += fn(b - A x0) before the first iteration; the
+ *                       guess is kept only if it lowers ||b - A x0||
+ *   NGAP_STAGE_PRECOND  in the first precond_iters iterations the
+ *                       preconditioner is z = V(r) + fn(r - A V(r)), with
+ *                       flexible (Polak-Ribiere) PCG in either precision.
+ * A breakdown with the stage active drops the stage and restarts. The
+ * stopping rule is unchanged, so every solve still meets rcvg with the exact
+ * operator. fn == NULL removes the stage. */
+typedef int (*ngap_stage_fn)(void *ctx, const double *r, double *e, long long n, void *stream);
+#define NGAP_STAGE_GUESS 1
+#define NGAP_STAGE_PRECOND 2
+int ngap_set_stage(ngap_t *h, ngap_stage_fn fn, void *ctx, int flags, int precond_iters);
 
 /* Static message for a return code. */
 const char *ngap_strerror(int code);

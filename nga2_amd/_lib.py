@@ -17,14 +17,15 @@ FP64, MIXED = 0, 1
 
 OK = 0
 ERR_ARG, ERR_STENCIL, ERR_NOT_INIT, ERR_NOT_SETUP, ERR_METHOD, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
-ERR_HIP, ERR_NONFINITE, ERR_BREAKDOWN, ERR_ALLOC, ERR_COARSE = 1, 2, 3, 4, 5
+ERR_HIP, ERR_NONFINITE, ERR_BREAKDOWN, ERR_ALLOC, ERR_COARSE, ERR_STAGE = 1, 2, 3, 4, 5, 6
+STAGE_GUESS, STAGE_PRECOND = 1, 2
 STATUS_CONVERGED, STATUS_MAXIT = 0, 1
 
 K_SPMV, K_RBGS, K_RESTRICT, K_PROLONG, K_UPDATE, K_EXACT = 0, 1, 2, 3, 4, 5
 
 EXPORTS = [
     "ngap_default_opts", "ngap_create", "ngap_init", "ngap_setup", "ngap_setup_device", "ngap_solve",
-    "ngap_solve_device", "ngap_set_tolerance", "ngap_destroy", "ngap_last_stats", "ngap_strerror",
+    "ngap_solve_device", "ngap_set_tolerance", "ngap_set_stage", "ngap_residual_device", "ngap_apply_device", "ngap_destroy", "ngap_last_stats", "ngap_strerror",
 ]
 TEST_EXPORTS = [
     "ngapt_nlevels", "ngapt_level_info", "ngapt_level_coefs", "ngapt_apply", "ngapt_rbgs", "ngapt_resid_restrict",
@@ -45,14 +46,14 @@ class Opts(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("status", C.c_int), ("it", C.c_int), ("nlevels", C.c_int), ("coarse_singular", C.c_int),
-                ("precision", C.c_int), ("fallback", C.c_int), ("restarts", C.c_int), ("pad0", C.c_int),
+                ("precision", C.c_int), ("fallback", C.c_int), ("restarts", C.c_int), ("stage_calls", C.c_int),
                 ("rerr", C.c_double), ("true_rerr", C.c_double), ("aerr", C.c_double),
                 ("cnorm_ratio", C.c_double), ("asym", C.c_double), ("setup_ms", C.c_double),
                 ("solve_ms", C.c_double), ("copy_ms", C.c_double), ("cells", C.c_longlong),
                 ("device_bytes", C.c_longlong)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad0"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 _dp = C.POINTER(C.c_double)
@@ -77,6 +78,9 @@ def lib():
     L.ngap_solve.argtypes = [_vp, _vp, _vp, _ip, _dp, _dp]
     L.ngap_solve_device.argtypes = [_vp, _vp, _vp, _ip, _dp, _dp]
     L.ngap_set_tolerance.argtypes = [_vp, C.c_int, C.c_double]
+    L.ngap_set_stage.argtypes = [_vp, _vp, _vp, C.c_int, C.c_int]
+    L.ngap_residual_device.argtypes = [_vp, _vp, _vp, _vp, _dp]
+    L.ngap_apply_device.argtypes = [_vp, _vp, _vp]
     L.ngap_destroy.argtypes = [_vp]
     L.ngap_last_stats.argtypes = [_vp, C.POINTER(Stats)]
     L.ngap_strerror.argtypes = [C.c_int]

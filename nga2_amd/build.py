@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 HIP_SOURCES = ["ngap_level.hip", "ngap_smooth.hip", "ngap_api.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-shared", "-std=c++17",
-    # packed FP32 instructions stay off (they have given silent wrong results on this pool)
+    # packed FP32 instructions stay off: some of their forms have returned wrong results without an error
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 
@@ -44,6 +44,20 @@ def build_solver_lib():
     out = os.path.join(LIBDIR, "libngapsolve.so")
     tmp = out + ".tmp"
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-I" + CSRC] + srcs + ["-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+NET_SOURCES = ["ngap_net.hip", "ngap_netmodel.hip"]
+
+
+def build_net_lib():
+    """libngapnet.so: the U-Net's BF16 MFMA convolution and training kernels."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libngapnet.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in NET_SOURCES]
     _run([hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-I" + CSRC] + srcs + ["-o", tmp])
     os.replace(tmp, out)
     return out
@@ -84,7 +98,7 @@ def build_fortran():
 
 
 def build_all(fortran=True):
-    out = {"solver": build_solver_lib(), "oracle": build_oracle()}
+    out = {"solver": build_solver_lib(), "net": build_net_lib(), "oracle": build_oracle()}
     if fortran:
         out["fortran"] = build_fortran()
     return out

@@ -251,6 +251,16 @@ __global__ void __launch_bounds__(NT) k_zero64(long long n, double *__restrict__
     for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) d[i] = 0.0;
 }
 
+// t = r - t
+__global__ void __launch_bounds__(NT) k_sub_from(long long n, const double *__restrict__ r, double *__restrict__ t) {
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) t[i] = r[i] - t[i];
+}
+
+// z += e
+__global__ void __launch_bounds__(NT) k_add(long long n, double *__restrict__ z, const double *__restrict__ e) {
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) z[i] += e[i];
+}
+
 __device__ __forceinline__ double sum_parts(const double *__restrict__ parts, int n, double *sm) {
     double s = 0.0;
     for (int q = threadIdx.x; q < n; q += NT) s += parts[q];
@@ -373,6 +383,11 @@ struct ngap_t {
     ngap_stats st{};
     long long dev_bytes = 0;
     std::vector<void *> base_allocs;
+    // correction stage (ngap_set_stage)
+    ngap_stage_fn stage_fn = nullptr;
+    void *stage_ctx = nullptr;
+    int stage_flags = 0, stage_iters = 0;
+    double *st_t = nullptr, *st_e = nullptr;  // stage input and output, allocated on first use
 };
 
 namespace {
@@ -462,6 +477,9 @@ void free_all(ngap_t *h) {
     }
     dfree(h, h->r32, sizeof(float) * N);
     dfree(h, h->z32, sizeof(float) * N);
+    dfree(h, h->st_t, sizeof(double) * N);
+    dfree(h, h->st_e, sizeof(double) * N);
+    h->st_t = h->st_e = nullptr;
     dfree(h, h->parts, sizeof(double) * 4 * MAX_PARTS);
     dfree(h, h->scal, sizeof(double) * NSCAL_ALLOC);
     dfree(h, h->flags, sizeof(int) * 4);
@@ -891,12 +909,43 @@ void precond(ngap_t *h, bool mixed) {
         vcycle<double>(h, h->r, h->z);
 }
 
+int stage_bufs(ngap_t *h) {
+    const size_t N = (size_t)h->g0.N;
+    if (!h->st_t && dmalloc(h, (void **)&h->st_t, sizeof(double) * N)) return NGAP_ERR_ALLOC;
+    if (!h->st_e && dmalloc(h, (void **)&h->st_e, sizeof(double) * N)) return NGAP_ERR_ALLOC;
+    return 0;
+}
+
+// z = V(r), plus the correction stage z += fn(r - A z) when `stage`. *z64
+// tells where the result is: h->z (FP64) or h->z32 (mixed without stage).
+int precond_stage(ngap_t *h, bool mixed, bool stage, bool *z64) {
+    precond(h, mixed);
+    *z64 = !mixed;
+    if (!stage) return 0;
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    if (mixed) launch_to_f64(s, N, h->z32, h->z);
+    Level &L0 = h->lev[0];
+    launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->z, h->st_t, h->parts + 2 * MAX_PARTS);
+    hipLaunchKernelGGL(k_sub_from, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r, h->st_t);
+    HIPCHK(hipGetLastError());
+    if (h->stage_fn(h->stage_ctx, h->st_t, h->st_e, N, (void *)s)) return NGAP_ERR_STAGE;
+    hipLaunchKernelGGL(k_add, dim3(nb), dim3(NT), 0, s, N, h->z, (const double *)h->st_e);
+    HIPCHK(hipGetLastError());
+    h->st.stage_calls++;
+    *z64 = true;
+    return 0;
+}
+
 // r, z -> rho (restart: p = z)
-void restart_dir(ngap_t *h, bool mixed, bool first) {
+int restart_dir(ngap_t *h, bool mixed, bool first, bool stage) {
     long long N = h->g0.N;
     int nb = grid_for(N);
-    precond(h, mixed);
-    if (mixed)
+    bool z64;
+    int rc = precond_stage(h, mixed, stage, &z64);
+    if (rc) return rc;
+    if (!z64)
         hipLaunchKernelGGL(k_dot_rz<float>, dim3(nb), dim3(NT), 0, h->stream, N, (const double *)h->r,
                            (const float *)h->z32, (const double *)nullptr, h->parts, (double *)nullptr);
     else
@@ -904,12 +953,44 @@ void restart_dir(ngap_t *h, bool mixed, bool first) {
                            (const double *)h->z, (const double *)nullptr, h->parts, (double *)nullptr);
     hipLaunchKernelGGL(k_fin_rho, dim3(1), dim3(NT), 0, h->stream, (const double *)h->parts, nb, h->scal, h->flags,
                        first ? 1 : 0);
-    if (mixed)
+    if (!z64)
         hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, h->stream, N, h->p, (const float *)h->z32,
                            (const double *)h->scal, 1);
     else
         hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, h->stream, N, h->p, (const double *)h->z,
                            (const double *)h->scal, 1);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// Guess stage: x += fn(r), kept only when it lowers the exact residual.
+// On entry r = b - A x with ||r||/||b|| = *rel; on exit r and *rel match x.
+int guess_stage(ngap_t *h, double bn, double *rel) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    int rc;
+    if ((rc = stage_bufs(h))) return rc;
+    if (h->stage_fn(h->stage_ctx, h->r, h->st_e, N, (void *)s)) return NGAP_ERR_STAGE;
+    h->st.stage_calls++;
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(NT), 0, s, N, (const double *)h->x, h->q);
+    hipLaunchKernelGGL(k_add, dim3(nb), dim3(NT), 0, s, N, h->x, (const double *)h->st_e);
+    exact_residual(h, h->r, S_RR);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scal(h))) return rc;
+    const double rel1 = std::sqrt(h->h_scal[S_RR]) / bn;
+    if (rel1 <= *rel) {
+        *rel = rel1;
+        return 0;
+    }
+    // rejected (larger or non-finite residual): back to the caller's guess
+    hipLaunchKernelGGL(k_copy, dim3(nb), dim3(NT), 0, s, N, (const double *)h->q, h->x);
+    exact_residual(h, h->r, S_RR);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scal(h))) return rc;
+    *rel = std::sqrt(h->h_scal[S_RR]) / bn;
+    h->st.fallback |= 2;
+    return 0;
 }
 
 int run_pcg(ngap_t *h) {
@@ -917,10 +998,14 @@ int run_pcg(ngap_t *h) {
     const int nb = grid_for(N);
     hipStream_t s = h->stream;
     bool mixed = h->opts.precision == NGAP_MIXED;
-    const bool flexible = mixed;
+    bool stage_pc = h->stage_fn && (h->stage_flags & NGAP_STAGE_PRECOND) && h->stage_iters > 0;
+    // a stage makes the preconditioner vary between iterations: flexible beta
+    const bool flexible = mixed || stage_pc;
     int rc;
     h->st.fallback = 0;
     h->st.restarts = 0;
+    h->st.stage_calls = 0;
+    if (stage_pc && (rc = stage_bufs(h))) return rc;
     hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, h->flags);
     exact_residual(h, h->r, S_RR);
     hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(NT), 0, s, N, (const double *)h->b, h->parts + MAX_PARTS);
@@ -942,6 +1027,9 @@ int run_pcg(ngap_t *h) {
     }
     double rel = std::sqrt(h->h_scal[S_RR]) / bn;
     const double tol = h->rcvg;
+    if (h->stage_fn && (h->stage_flags & NGAP_STAGE_GUESS) && rel > tol && std::isfinite(rel)) {
+        if ((rc = guess_stage(h, bn, &rel))) return rc;
+    }
     int it = 0;
     bool restart = true, first = true, converged = false;
     double tr = rel;
@@ -964,7 +1052,7 @@ int run_pcg(ngap_t *h) {
         }
         if (it >= h->maxit) break;
         if (restart) {
-            restart_dir(h, mixed, first);
+            if ((rc = restart_dir(h, mixed, first, stage_pc && it < h->stage_iters))) return rc;
             first = false;
             restart = false;
         }
@@ -984,9 +1072,16 @@ int run_pcg(ngap_t *h) {
         if (h->h_flags[0]) {
             // preconditioner or step breakdown: x, r untouched by a zero step
             hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, h->flags);
+            if (stage_pc) {
+                // the stage made the preconditioner indefinite: drop it first
+                stage_pc = false;
+                h->st.fallback |= 2;
+                restart = true;
+                continue;
+            }
             if (mixed) {
                 mixed = false;
-                h->st.fallback = 1;
+                h->st.fallback |= 1;
                 restart = true;
                 continue;
             }
@@ -998,8 +1093,9 @@ int run_pcg(ngap_t *h) {
         }
         if (!std::isfinite(rel)) return NGAP_ERR_BREAKDOWN;
         if (rel <= tol || it >= h->maxit) continue;
-        precond(h, mixed);
-        if (mixed)
+        bool z64;
+        if ((rc = precond_stage(h, mixed, stage_pc && it < h->stage_iters, &z64))) return rc;
+        if (!z64)
             hipLaunchKernelGGL(k_dot_rz<float>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r,
                                (const float *)h->z32, flexible ? (const double *)h->q : (const double *)nullptr,
                                h->parts, h->parts + MAX_PARTS);
@@ -1009,7 +1105,7 @@ int run_pcg(ngap_t *h) {
                                h->parts, h->parts + MAX_PARTS);
         hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, (const double *)h->parts,
                            (const double *)(h->parts + MAX_PARTS), nb, h->scal, h->flags, flexible ? 1 : 0);
-        if (mixed)
+        if (!z64)
             hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, s, N, h->p, (const float *)h->z32,
                                (const double *)h->scal, 0);
         else
@@ -1270,6 +1366,7 @@ const char *ngap_strerror(int code) {
         case NGAP_ERR_BREAKDOWN: return "PCG breakdown";
         case NGAP_ERR_ALLOC: return "allocation failed";
         case NGAP_ERR_COARSE: return "coarsest-level factorisation failed";
+        case NGAP_ERR_STAGE: return "correction stage failed";
         default: return "unknown error";
     }
 }
@@ -1355,6 +1452,47 @@ int ngap_solve(ngap_t *h, const double *rhs, double *sol, int *it, double *rerr,
 
 int ngap_solve_device(ngap_t *h, const double *d_rhs, double *d_sol, int *it, double *rerr, double *aerr) {
     return do_solve(h, d_rhs, d_sol, true, it, rerr, aerr);
+}
+
+int ngap_residual_device(ngap_t *h, const double *d_rhs, const double *d_sol, double *d_res, double *rnorm) {
+    if (!h || !d_rhs || !d_sol || !d_res) return NGAP_ERR_ARG;
+    if (!h->setup_done) return NGAP_ERR_NOT_SETUP;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipDeviceSynchronize());
+    hipStream_t s = h->stream;
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, d_rhs, (const unsigned char *)h->part, h->b,
+                       (double *)nullptr);
+    hipLaunchKernelGGL(k_gather, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, d_sol, (const unsigned char *)h->part, h->x,
+                       (double *)nullptr);
+    exact_residual(h, d_res, S_RRT);
+    HIPCHK(hipGetLastError());
+    int rc = read_scal(h);
+    if (rc) return rc;
+    if (rnorm) *rnorm = std::sqrt(h->h_scal[S_RRT]);
+    return 0;
+}
+
+int ngap_apply_device(ngap_t *h, const double *d_x, double *d_y) {
+    if (!h || !d_x || !d_y) return NGAP_ERR_ARG;
+    if (!h->setup_done) return NGAP_ERR_NOT_SETUP;
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipDeviceSynchronize());
+    Level &L0 = h->lev[0];
+    launch_apply<double>(h->stream, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, d_x, d_y, h->parts + 2 * MAX_PARTS);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+int ngap_set_stage(ngap_t *h, ngap_stage_fn fn, void *ctx, int flags, int precond_iters) {
+    if (!h || precond_iters < 0 || (flags & ~(NGAP_STAGE_GUESS | NGAP_STAGE_PRECOND))) return NGAP_ERR_ARG;
+    h->stage_fn = fn;
+    h->stage_ctx = fn ? ctx : nullptr;
+    h->stage_flags = fn ? flags : 0;
+    h->stage_iters = fn ? precond_iters : 0;
+    return 0;
 }
 
 int ngap_set_tolerance(ngap_t *h, int maxit, double rcvg) {

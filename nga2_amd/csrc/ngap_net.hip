@@ -1,0 +1,531 @@
+// Kernels of the small 3-D U-Net that supplies initial guesses and a
+// flexible-PCG stage to the pressure solver (gfx950).
+//
+// Activations are channels-last: a[v][c], v = x + nx (y + ny z), BF16 for the
+// convolution inputs, FP32 elsewhere. Padding is periodic (the workloads the
+// network serves are periodic boxes). 3x3x3 convolutions are implicit GEMMs
+// on BF16 MFMA (v_mfma_f32_16x16x32_bf16) with FP32 accumulation:
+//   out[v][co] = sum_tap sum_ci W[tap][co][ci] * in[v + off(tap)][ci],
+// tap = (dz+1)*9 + (dy+1)*3 + (dx+1), off = (dx, dy, dz) in {-1,0,1}^3.
+// Weight layout [27][Cout][Cin] (ci fastest) makes both MFMA operands 16-byte
+// LDS reads. Channel counts of MFMA layers are multiples of 32; the first
+// layer (1-2 input channels) and the 1x1 output layer use plain FP32 kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "ngapnet.h"
+
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// output tile of one workgroup: 16 x 4 x 4 voxels, 4 waves (one z-slice each)
+constexpr int TXV = 16, TYV = 4, TZV = 4;
+constexpr int HX = TXV + 2, HY = TYV + 2, HZ = TZV + 2;
+constexpr int HVOX = HX * HY * HZ;  // 648 halo voxels
+constexpr int CK = 32;              // channel chunk
+constexpr int IN_STRIDE = 40;       // bf16 per halo voxel row in LDS (80 B: 16-B aligned, fewer bank conflicts)
+constexpr int W_STRIDE = 40;        // bf16 per weight row (co) in LDS
+
+// ---------------------------------------------------------------- forward 3x3x3
+// grid.x = tiles, grid.y = Cout/32. Writes FP32 pre-activations (out32, may be
+// null) and/or BF16 ReLU(out) (outb, may be null).
+__global__ void __launch_bounds__(256) k_conv3_fwd(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wt,
+                                                   float *__restrict__ out32, __bf16 *__restrict__ outb, int nx,
+                                                   int ny, int nz, int cin, int cout, int relu) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_in[HVOX * IN_STRIDE];
+    __shared__ __attribute__((aligned(16))) __bf16 s_w[27 * CK * W_STRIDE];
+    const int tilesx = nx / TXV, tilesy = ny / TYV;
+    const int t = blockIdx.x;
+    const int x0 = (t % tilesx) * TXV;
+    const int y0 = ((t / tilesx) % tilesy) * TYV;
+    const int z0 = (t / (tilesx * tilesy)) * TZV;
+    const int co0 = blockIdx.y * CK;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    f32x4 acc[TYV][2];
+    for (int a = 0; a < TYV; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int lrow = lane & 15, lk = (lane >> 4) * 8;
+    for (int ci0 = 0; ci0 < cin; ci0 += CK) {
+        // stage the halo tile: 648 voxels x 32 channels, 16 B per piece
+        for (int q = tid; q < HVOX * 4; q += 256) {
+            const int hv = q >> 2, part = q & 3;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+            const long long gv = (long long)gx + (long long)nx * (gy + (long long)ny * gz);
+            const uint4 val = *reinterpret_cast<const uint4 *>(in + gv * cin + ci0 + part * 8);
+            *reinterpret_cast<uint4 *>(s_in + hv * IN_STRIDE + part * 8) = val;
+        }
+        // stage the weights of this ci chunk: 27 taps x 32 co x 32 ci
+        for (int q = tid; q < 27 * CK * 4; q += 256) {
+            const int row = q >> 2, part = q & 3;  // row = tap*32 + co
+            const int tap = row / CK, co = row % CK;
+            const uint4 val =
+                *reinterpret_cast<const uint4 *>(wt + ((long long)tap * cout + co0 + co) * cin + ci0 + part * 8);
+            *reinterpret_cast<uint4 *>(s_w + row * W_STRIDE + part * 8) = val;
+        }
+        __syncthreads();
+        for (int tap = 0; tap < 27; tap++) {
+            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // halo offsets (0..2)
+            bf16x8 bfr[2];
+            for (int nb = 0; nb < 2; nb++)
+                bfr[nb] = *reinterpret_cast<const bf16x8 *>(s_w + (tap * CK + nb * 16 + lrow) * W_STRIDE + lk);
+            for (int mb = 0; mb < TYV; mb++) {
+                const int hv = (lrow + dx) + HX * ((mb + dy) + HY * (wave + dz));
+                const bf16x8 afr = *reinterpret_cast<const bf16x8 *>(s_in + hv * IN_STRIDE + lk);
+                for (int nb = 0; nb < 2; nb++)
+                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nb], acc[mb][nb], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds D[m = 4*(lane/16)+q][n = lane%16] of each 16x16 block
+    for (int mb = 0; mb < TYV; mb++)
+        for (int nb = 0; nb < 2; nb++)
+            for (int q = 0; q < 4; q++) {
+                const int vx = x0 + 4 * (lane >> 4) + q, vy = y0 + mb, vz = z0 + wave;
+                const long long v = (long long)vx + (long long)nx * (vy + (long long)ny * vz);
+                const int co = co0 + nb * 16 + lrow;
+                const float val = acc[mb][nb][q];
+                if (out32) out32[v * cout + co] = val;
+                if (outb) outb[v * cout + co] = (__bf16)(relu ? fmaxf(val, 0.f) : val);
+            }
+}
+
+// ---------------------------------------------------------------- weight gradient
+// dW[tap][co][ci] = sum_v dout[v][co] * in[v + off(tap)][ci], split over voxel
+// tiles: workgroup (g, cob, cib) sums tiles g, g + G, g + 2G, ... into
+// part[g][27][32][32] (co-major within the block). Four waves share the taps.
+constexpr int KT = TXV * TYV * TZV;  // 256 voxels per tile
+constexpr int DT_STRIDE = KT + 8;    // bf16 per co row of the transposed dout tile
+constexpr int IT_STRIDE = HVOX + 8;  // bf16 per ci row of the transposed input tile
+
+__global__ void __launch_bounds__(256) k_conv3_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ dout,
+                                                     float *__restrict__ part, int nx, int ny, int nz, int cin,
+                                                     int cout, int G) {
+    __shared__ __attribute__((aligned(16))) __bf16 s_inT[CK * IT_STRIDE];
+    __shared__ __attribute__((aligned(16))) __bf16 s_dT[CK * DT_STRIDE];
+    const int tilesx = nx / TXV, tilesy = ny / TYV, tilesz = nz / TZV;
+    const int ntiles = tilesx * tilesy * tilesz;
+    const int g = blockIdx.x, cob = blockIdx.y, cib = blockIdx.z;
+    const int co0 = cob * CK, ci0 = cib * CK;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lrow = lane & 15, lq = lane >> 4;
+    f32x4 acc[7][2][2];
+    for (int a = 0; a < 7; a++)
+        for (int b = 0; b < 2; b++)
+            for (int c = 0; c < 2; c++) acc[a][b][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int t = g; t < ntiles; t += G) {
+        const int x0 = (t % tilesx) * TXV;
+        const int y0 = ((t / tilesx) % tilesy) * TYV;
+        const int z0 = (t / (tilesx * tilesy)) * TZV;
+        // transposed input halo tile: s_inT[ci][hv]
+        for (int q = tid; q < HVOX * CK; q += 256) {
+            const int hv = q / CK, c = q % CK;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
+            const long long gv = (long long)gx + (long long)nx * (gy + (long long)ny * gz);
+            s_inT[c * IT_STRIDE + hv] = in[gv * cin + ci0 + c];
+        }
+        // transposed dout tile: s_dT[co][v]
+        for (int q = tid; q < KT * CK; q += 256) {
+            const int v = q / CK, c = q % CK;
+            const int vx = x0 + v % TXV, vy = y0 + (v / TXV) % TYV, vz = z0 + v / (TXV * TYV);
+            const long long gv = (long long)vx + (long long)nx * (vy + (long long)ny * vz);
+            s_dT[c * DT_STRIDE + v] = dout[gv * cout + co0 + c];
+        }
+        __syncthreads();
+        for (int kc = 0; kc < KT / 32; kc++) {
+            // voxels of this K chunk: v = 32*kc + 8*lq + e, e = 0..7 (consecutive x)
+            const int vb = 32 * kc + 8 * lq;
+            const int vx = vb % TXV, vy = (vb / TXV) % TYV, vz = vb / (TXV * TYV);
+            bf16x8 afr[2];
+            for (int mb = 0; mb < 2; mb++)
+                afr[mb] = *reinterpret_cast<const bf16x8 *>(s_dT + (mb * 16 + lrow) * DT_STRIDE + vb);
+            for (int tt = 0; tt < 7; tt++) {
+                const int tap = wave + 4 * tt;
+                if (tap >= 27) break;
+                const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+                const int hv = (vx + dx) + HX * ((vy + dy) + HY * (vz + dz));
+                for (int nb = 0; nb < 2; nb++) {
+                    const __bf16 *src = s_inT + (nb * 16 + lrow) * IT_STRIDE + hv;
+                    bf16x8 bfr;
+                    for (int e = 0; e < 8; e++) bfr[e] = src[e];
+                    for (int mb = 0; mb < 2; mb++)
+                        acc[tt][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[mb], bfr, acc[tt][mb][nb], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // D[m = co][n = ci]: lane holds m = 4*lq + q, n = lrow
+    float *dst = part + (long long)((g * gridDim.z + cib) * gridDim.y + cob) * 27 * CK * CK;
+    for (int tt = 0; tt < 7; tt++) {
+        const int tap = wave + 4 * tt;
+        if (tap >= 27) break;
+        for (int mb = 0; mb < 2; mb++)
+            for (int nb = 0; nb < 2; nb++)
+                for (int q = 0; q < 4; q++) {
+                    const int co = mb * 16 + 4 * lq + q, ci = nb * 16 + lrow;
+                    dst[(tap * CK + co) * CK + ci] = acc[tt][mb][nb][q];
+                }
+    }
+}
+
+// dW[tap][co][ci] = sum_g part[g][cib][cob][tap][co%32][ci%32] in fixed order
+__global__ void __launch_bounds__(256) k_wgrad_reduce(const float *__restrict__ part, float *__restrict__ dw, int cin,
+                                                      int cout, int G) {
+    const long long n = 27LL * cout * cin;
+    const int ncb = cin / CK, nob = cout / CK;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ci = (int)(i % cin);
+        const int co = (int)((i / cin) % cout);
+        const int tap = (int)(i / ((long long)cin * cout));
+        const int cib = ci / CK, cob = co / CK;
+        const long long off = (long long)(tap * CK + co % CK) * CK + ci % CK;
+        float s = 0.f;
+        for (int g = 0; g < G; g++) s += part[(long long)((g * ncb + cib) * nob + cob) * 27 * CK * CK + off];
+        dw[i] = s;
+    }
+}
+
+// weights of the data gradient: wd[tap'][ci][co] = wt[26 - tap'][co][ci]
+__global__ void k_flip_transpose(const __bf16 *__restrict__ wt, __bf16 *__restrict__ wd, int cin, int cout) {
+    const long long n = 27LL * cin * cout;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int co = (int)(i % cout);
+        const int ci = (int)((i / cout) % cin);
+        const int tp = (int)(i / ((long long)cin * cout));
+        wd[i] = wt[((long long)(26 - tp) * cout + co) * cin + ci];
+    }
+}
+
+// ---------------------------------------------------------------- small-channel layers (FP32)
+// out[v][co] = sum_tap sum_ci w[tap][co][ci] in[v+off][ci]   (cin small)
+__global__ void k_conv3_small_fwd(const float *__restrict__ in, const float *__restrict__ w, float *__restrict__ out32,
+                                  __bf16 *__restrict__ outb, int nx, int ny, int nz, int cin, int cout, int relu) {
+    const long long nv = (long long)nx * ny * nz;
+    const long long i = blockIdx.x * 256LL + threadIdx.x;
+    if (i >= nv * cout) return;
+    const int co = (int)(i % cout);
+    const long long v = i / cout;
+    const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+    float s = 0.f;
+    for (int tap = 0; tap < 27; tap++) {
+        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+        const long long u = wrapi(x + dx, nx) + (long long)nx * (wrapi(y + dy, ny) + (long long)ny * wrapi(z + dz, nz));
+        for (int ci = 0; ci < cin; ci++) s += w[(tap * cout + co) * cin + ci] * in[u * cin + ci];
+    }
+    if (out32) out32[i] = s;
+    if (outb) outb[i] = (__bf16)(relu ? fmaxf(s, 0.f) : s);
+}
+
+// dW of the small layer: one block per weight, fixed-order block reduction
+__global__ void __launch_bounds__(256) k_conv3_small_wgrad(const float *__restrict__ in, const float *__restrict__ dout,
+                                                           float *__restrict__ dw, int nx, int ny, int nz, int cin,
+                                                           int cout) {
+    __shared__ float sm[4];
+    const int wi = blockIdx.x;  // tap*cout*cin + co*cin + ci
+    const int ci = wi % cin, co = (wi / cin) % cout, tap = wi / (cin * cout);
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const long long nv = (long long)nx * ny * nz;
+    float s = 0.f;
+    for (long long v = threadIdx.x; v < nv; v += 256) {
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        const long long u = wrapi(x + dx, nx) + (long long)nx * (wrapi(y + dy, ny) + (long long)ny * wrapi(z + dz, nz));
+        s += dout[v * cout + co] * in[u * cin + ci];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) dw[wi] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+
+// ---------------------------------------------------------------- elementwise
+__global__ void k_relu_bwd(const float *__restrict__ dy, const __bf16 *__restrict__ y, float *__restrict__ dx,
+                           __bf16 *__restrict__ dxb, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const float g = (float)y[i] > 0.f ? dy[i] : 0.f;
+        if (dx) dx[i] = g;
+        if (dxb) dxb[i] = (__bf16)g;
+    }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ s, __bf16 *__restrict__ d, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = (__bf16)s[i];
+}
+
+// average pool 2x2x2 (bf16 -> bf16), fine dims nx,ny,nz
+__global__ void k_pool2(const __bf16 *__restrict__ in, __bf16 *__restrict__ out, int nx, int ny, int nz, int c) {
+    const int mx = nx / 2, my = ny / 2, mz = nz / 2;
+    const long long n = (long long)mx * my * mz * c;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const long long v = i / c;
+        const int x = (int)(v % mx), y = (int)((v / mx) % my), z = (int)(v / ((long long)mx * my));
+        float s = 0.f;
+        for (int q = 0; q < 8; q++) {
+            const long long u = (2 * x + (q & 1)) + (long long)nx * ((2 * y + ((q >> 1) & 1)) + (long long)ny * (2 * z + (q >> 2)));
+            s += (float)in[u * c + ch];
+        }
+        out[i] = (__bf16)(0.125f * s);
+    }
+}
+
+// d(in) of the pool: each fine voxel gets dout(coarse)/8 (fp32 -> fp32, added to dst when acc)
+__global__ void k_pool2_bwd(const float *__restrict__ dout, float *__restrict__ din, int nx, int ny, int nz, int c,
+                            int acc) {
+    const long long n = (long long)nx * ny * nz * c;
+    const int mx = nx / 2, my = ny / 2;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const long long v = i / c;
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        const long long u = (x / 2) + (long long)mx * ((y / 2) + (long long)my * (z / 2));
+        const float g = 0.125f * dout[u * c + ch];
+        din[i] = acc ? din[i] + g : g;
+    }
+}
+
+// out[fine][c0 + ch] = in[coarse][ch] (nearest-neighbour upsampling into a
+// channel slice of a concatenation buffer with ctot channels)
+__global__ void k_up2(const __bf16 *__restrict__ in, __bf16 *__restrict__ out, int nx, int ny, int nz, int c, int ctot,
+                      int c0) {
+    const long long n = (long long)nx * ny * nz * c;
+    const int mx = nx / 2, my = ny / 2;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const long long v = i / c;
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        const long long u = (x / 2) + (long long)mx * ((y / 2) + (long long)my * (z / 2));
+        out[v * ctot + c0 + ch] = in[u * c + ch];
+    }
+}
+
+// d(coarse) = sum over the 8 fine children of d(fine)[c0 + ch]
+__global__ void k_up2_bwd(const float *__restrict__ dout, float *__restrict__ din, int nx, int ny, int nz, int c,
+                          int ctot, int c0) {
+    const int mx = nx / 2, my = ny / 2, mz = nz / 2;
+    const long long n = (long long)mx * my * mz * c;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const long long v = i / c;
+        const int x = (int)(v % mx), y = (int)((v / mx) % my), z = (int)(v / ((long long)mx * my));
+        float s = 0.f;
+        for (int q = 0; q < 8; q++) {
+            const long long u = (2 * x + (q & 1)) + (long long)nx * ((2 * y + ((q >> 1) & 1)) + (long long)ny * (2 * z + (q >> 2)));
+            s += dout[u * ctot + c0 + ch];
+        }
+        din[i] = s;
+    }
+}
+
+// copy a channel slice: dst[v][d0 + ch] = src[v][s0 + ch], ch < c (bf16 or fp32)
+template <typename T>
+__global__ void k_chan_copy(const T *__restrict__ src, T *__restrict__ dst, long long nv, int c, int cs, int s0, int cd,
+                            int d0) {
+    const long long n = nv * c;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int ch = (int)(i % c);
+        const long long v = i / c;
+        dst[v * cd + d0 + ch] = src[v * cs + s0 + ch];
+    }
+}
+
+// y[v] = sum_c w[c] x[v][c] (1x1 output layer, x bf16)
+__global__ void k_conv1_fwd(const __bf16 *__restrict__ x, const float *__restrict__ w, float *__restrict__ y,
+                            long long nv, int c) {
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
+        float s = 0.f;
+        for (int ch = 0; ch < c; ch++) s += w[ch] * (float)x[v * c + ch];
+        y[v] = s;
+    }
+}
+
+// dx[v][c] = w[c] dy[v]; dW partials per block: part[b][c] = sum_v dy[v] x[v][c]
+__global__ void __launch_bounds__(256) k_conv1_bwd(const __bf16 *__restrict__ x, const float *__restrict__ w,
+                                                   const float *__restrict__ dy, float *__restrict__ dx,
+                                                   float *__restrict__ part, long long nv, int c) {
+    __shared__ float sm[4][64];
+    float accw[64];
+    for (int ch = 0; ch < c; ch++) accw[ch] = 0.f;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
+        const float g = dy[v];
+        for (int ch = 0; ch < c; ch++) {
+            dx[v * c + ch] = w[ch] * g;
+            accw[ch] += g * (float)x[v * c + ch];
+        }
+    }
+    for (int ch = 0; ch < c; ch++) {
+        float s = accw[ch];
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+        if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6][ch] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < c) part[blockIdx.x * c + threadIdx.x] = sm[0][threadIdx.x] + sm[1][threadIdx.x] + sm[2][threadIdx.x] + sm[3][threadIdx.x];
+}
+
+__global__ void k_sum_parts(const float *__restrict__ part, float *__restrict__ out, int nb, int c) {
+    const int ch = threadIdx.x;
+    if (ch >= c) return;
+    float s = 0.f;
+    for (int b = 0; b < nb; b++) s += part[b * c + ch];
+    out[ch] = s;
+}
+
+// Adam on FP32 master weights; writes the BF16 copy
+__global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       __bf16 *__restrict__ wb, long long n, float lr, float b1, float b2, float eps, float c1,
+                       float c2, float scale) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const float gi = g[i] * scale;
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (wb) wb[i] = (__bf16)wi;
+    }
+}
+
+int blocks_for(long long n) {
+    long long b = (n + 255) / 256;
+    if (b > 16384) b = 16384;
+    return (int)(b < 1 ? 1 : b);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ngapnet_conv3_fwd(const void *in, const void *wt, float *out32, void *outb, int nx, int ny, int nz, int cin,
+                      int cout, int relu, void *stream) {
+    if (cin % CK || cout % CK || nx % TXV || ny % TYV || nz % TZV || cin <= 0 || cout <= 0) return -1;
+    dim3 grid((nx / TXV) * (ny / TYV) * (nz / TZV), cout / CK);
+    hipLaunchKernelGGL(k_conv3_fwd, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)in, (const __bf16 *)wt,
+                       out32, (__bf16 *)outb, nx, ny, nz, cin, cout, relu);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_wgrad_parts(int nx, int ny, int nz, int cin, int cout) {
+    const int ntiles = (nx / TXV) * (ny / TYV) * (nz / TZV);
+    int G = ntiles < 256 ? ntiles : 256;
+    (void)cin;
+    (void)cout;
+    return G;
+}
+
+int ngapnet_conv3_wgrad(const void *in, const void *dout, float *dw, float *part, int nx, int ny, int nz, int cin,
+                        int cout, void *stream) {
+    if (cin % CK || cout % CK || nx % TXV || ny % TYV || nz % TZV) return -1;
+    const int G = ngapnet_wgrad_parts(nx, ny, nz, cin, cout);
+    dim3 grid(G, cout / CK, cin / CK);
+    hipLaunchKernelGGL(k_conv3_wgrad, grid, dim3(256), 0, (hipStream_t)stream, (const __bf16 *)in,
+                       (const __bf16 *)dout, part, nx, ny, nz, cin, cout, G);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3(blocks_for(27LL * cin * cout)), dim3(256), 0, (hipStream_t)stream,
+                       (const float *)part, dw, cin, cout, G);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_flip_transpose(const void *wt, void *wd, int cin, int cout, void *stream) {
+    hipLaunchKernelGGL(k_flip_transpose, dim3(blocks_for(27LL * cin * cout)), dim3(256), 0, (hipStream_t)stream,
+                       (const __bf16 *)wt, (__bf16 *)wd, cin, cout);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_conv3_small_fwd(const float *in, const float *w, float *out32, void *outb, int nx, int ny, int nz,
+                            int cin, int cout, int relu, void *stream) {
+    const long long n = (long long)nx * ny * nz * cout;
+    hipLaunchKernelGGL(k_conv3_small_fwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in, w,
+                       out32, (__bf16 *)outb, nx, ny, nz, cin, cout, relu);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, int nx, int ny, int nz, int cin, int cout,
+                              void *stream) {
+    hipLaunchKernelGGL(k_conv3_small_wgrad, dim3(27 * cin * cout), dim3(256), 0, (hipStream_t)stream, in, dout, dw,
+                       nx, ny, nz, cin, cout);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_relu_bwd(const float *dy, const void *y, float *dx, void *dxb, long long n, void *stream) {
+    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, dy, (const __bf16 *)y, dx,
+                       (__bf16 *)dxb, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_to_bf16(const float *s, void *d, long long n, void *stream) {
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, s, (__bf16 *)d, n);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_pool2(const void *in, void *out, int nx, int ny, int nz, int c, void *stream) {
+    hipLaunchKernelGGL(k_pool2, dim3(blocks_for((long long)nx * ny * nz * c / 8)), dim3(256), 0, (hipStream_t)stream,
+                       (const __bf16 *)in, (__bf16 *)out, nx, ny, nz, c);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_pool2_bwd(const float *dout, float *din, int nx, int ny, int nz, int c, int acc, void *stream) {
+    hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for((long long)nx * ny * nz * c)), dim3(256), 0, (hipStream_t)stream,
+                       dout, din, nx, ny, nz, c, acc);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_up2(const void *in, void *out, int nx, int ny, int nz, int c, int ctot, int c0, void *stream) {
+    hipLaunchKernelGGL(k_up2, dim3(blocks_for((long long)nx * ny * nz * c)), dim3(256), 0, (hipStream_t)stream,
+                       (const __bf16 *)in, (__bf16 *)out, nx, ny, nz, c, ctot, c0);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_up2_bwd(const float *dout, float *din, int nx, int ny, int nz, int c, int ctot, int c0, void *stream) {
+    hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for((long long)nx * ny * nz * c / 8)), dim3(256), 0,
+                       (hipStream_t)stream, dout, din, nx, ny, nz, c, ctot, c0);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_chan_copy_bf16(const void *src, void *dst, long long nv, int c, int cs, int s0, int cd, int d0,
+                           void *stream) {
+    hipLaunchKernelGGL(k_chan_copy<__bf16>, dim3(blocks_for(nv * c)), dim3(256), 0, (hipStream_t)stream,
+                       (const __bf16 *)src, (__bf16 *)dst, nv, c, cs, s0, cd, d0);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_chan_copy_f32(const float *src, float *dst, long long nv, int c, int cs, int s0, int cd, int d0,
+                          void *stream) {
+    hipLaunchKernelGGL(k_chan_copy<float>, dim3(blocks_for(nv * c)), dim3(256), 0, (hipStream_t)stream, src, dst, nv,
+                       c, cs, s0, cd, d0);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_conv1_fwd(const void *x, const float *w, float *y, long long nv, int c, void *stream) {
+    hipLaunchKernelGGL(k_conv1_fwd, dim3(blocks_for(nv)), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, y,
+                       nv, c);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_conv1_bwd(const void *x, const float *w, const float *dy, float *dx, float *dw, float *part, long long nv,
+                      int c, void *stream) {
+    if (c > 64) return -1;
+    const int nb = 1024;
+    hipLaunchKernelGGL(k_conv1_bwd, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, dy, dx, part,
+                       nv, c);
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, (hipStream_t)stream, (const float *)part, dw, nb, c);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_adam(float *w, const float *g, float *m, float *v, void *wb, long long n, float lr, float b1, float b2,
+                 float eps, int step, float scale, void *stream) {
+    float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, (__bf16 *)wb, n, lr,
+                       b1, b2, eps, c1, c2, scale);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int ngapnet_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 1; }
+
+}  // extern "C"

@@ -17,6 +17,9 @@ module ngap_c
    ! precision modes
    integer, parameter, public :: ngap_fp64=0
    integer, parameter, public :: ngap_mixed=1
+   ! correction stage flags (ngap_set_stage)
+   integer, parameter, public :: ngap_stage_guess=1
+   integer, parameter, public :: ngap_stage_precond=2
 
    ! status of the last solve
    integer, parameter, public :: ngap_status_converged=0
@@ -34,12 +37,12 @@ module ngap_c
    end type ngap_opts_t
 
    type, bind(C), public :: ngap_stats_t
-      integer(c_int) :: status,it,nlevels,coarse_singular,precision,fallback,restarts,pad0
+      integer(c_int) :: status,it,nlevels,coarse_singular,precision,fallback,restarts,stage_calls
       real(c_double) :: rerr,true_rerr,aerr,cnorm_ratio,asym,setup_ms,solve_ms,copy_ms
       integer(c_long_long) :: cells,device_bytes
    end type ngap_stats_t
 
-   public :: ngap_default_opts,ngap_create,ngap_init,ngap_setup,ngap_solve,ngap_set_tolerance
+   public :: ngap_default_opts,ngap_create,ngap_init,ngap_setup,ngap_solve,ngap_set_tolerance,ngap_set_stage
    public :: ngap_destroy,ngap_last_stats,ngap_strerror_f
 
    interface
@@ -82,6 +85,15 @@ module ngap_c
          integer(c_int), value :: maxit
          real(c_double), value :: rcvg
       end function ngap_set_tolerance
+      !> Correction stage (learned guess / flexible-PCG stage): fn is c_funloc of a
+      !> bind(C) function with the signature of ngap_stage_fn, e.g. ngapnet_stage
+      integer(c_int) function ngap_set_stage(h,fn,ctx,flags,precond_iters) bind(C,name='ngap_set_stage')
+         import :: c_int,c_ptr,c_funptr
+         type(c_ptr), value :: h
+         type(c_funptr), value :: fn
+         type(c_ptr), value :: ctx
+         integer(c_int), value :: flags,precond_iters
+      end function ngap_set_stage
       integer(c_int) function ngap_destroy(h) bind(C,name='ngap_destroy')
          import :: c_int,c_ptr
          type(c_ptr), value :: h

@@ -297,6 +297,26 @@ class RhsSequence:
         _remove_mean_interior(self.cfg, r)
         return r
 
+    def rhs_device(self, step, device="cuda:0"):
+        """rhs(step) computed with torch's FFT on the device: the ghosted
+        array flattened in Fortran order (the layout of ngap_solve_device)."""
+        import torch
+
+        if getattr(self, "_dev", None) != str(device):
+            self._dcoef = torch.from_numpy(self.coef).to(device)
+            self._domega = torch.from_numpy(self.omega).to(device)
+            self._dvol = torch.from_numpy(np.ascontiguousarray(self.vol)).to(device)
+            self._dev = str(device)
+        c = self._dcoef * torch.exp(1j * self._domega * float(step))
+        f = torch.fft.ifftn(c).real * self.n ** 3
+        a = -self._dvol * f  # [x, y, z]
+        a = a - a.mean()
+        cfg = self.cfg
+        g = torch.zeros((cfg.nzo, cfg.nyo, cfg.nxo), dtype=torch.float64, device=device)
+        no = cfg.no
+        g[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx] = a.permute(2, 1, 0)
+        return g.reshape(-1)
+
 
 WORKLOADS = {
     "periodic_gaussian": periodic_gaussian,

@@ -301,6 +301,29 @@ class DeviceSolve:
     def set_tolerance(self, maxit, rcvg):
         check("ngap_set_tolerance", lib().ngap_set_tolerance(self._h, int(maxit), float(rcvg)))
 
+    def residual(self, d_rhs, d_sol, d_res):
+        """d_res (interior, x fastest) = rhs - A sol; returns ||res||_2."""
+        self._torch_sync(d_rhs)
+        nrm = C.c_double()
+        check("ngap_residual_device", lib().ngap_residual_device(self._h, C.c_void_p(d_rhs.data_ptr()),
+                                                                 C.c_void_p(d_sol.data_ptr()),
+                                                                 C.c_void_p(d_res.data_ptr()), C.byref(nrm)))
+        return nrm.value
+
+    def apply(self, d_x, d_y):
+        """d_y = A d_x over the interior cells (x fastest), FP64 device tensors."""
+        self._torch_sync(d_x)
+        check("ngap_apply_device", lib().ngap_apply_device(self._h, C.c_void_p(d_x.data_ptr()),
+                                                           C.c_void_p(d_y.data_ptr())))
+
+    def set_stage(self, stage, flags=_lib.STAGE_GUESS, precond_iters=0):
+        """Attach a correction stage, given as the (function pointer, context)
+        pair of unet.UNet.stage(); None removes it."""
+        fn, ctx = stage if stage is not None else (None, None)
+        self._stage_ref = stage  # the context must outlive the handle's use of it
+        check("ngap_set_stage", lib().ngap_set_stage(self._h, fn, ctx, int(flags) if fn else 0,
+                                                     int(precond_iters) if fn else 0))
+
     def stats(self):
         s = _lib.Stats()
         check("ngap_last_stats", lib().ngap_last_stats(self._h, C.byref(s)))

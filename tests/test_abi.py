@@ -28,6 +28,21 @@ def test_library_exports_all_declared_symbols():
     assert set(_lib.TEST_EXPORTS) <= set(names)
 
 
+def test_net_library_exports_all_declared_symbols():
+    from nga2_amd import unet
+
+    txt = open(os.path.join(ROOT, "include", "ngapnet.h")).read()
+    names = sorted(set(re.findall(r"^\s*int\s+(ngapnet_\w+)\s*\(", txt, re.M)))
+    assert len(names) >= 18
+    L = unet.lib()  # loads and sets every argtypes entry without touching the device
+    for n in names:
+        assert hasattr(L, n), n
+        assert getattr(L, n).argtypes is not None, n
+    # pure host query: voxel-tile groups of the weight-gradient split
+    assert L.ngapnet_wgrad_parts(16, 4, 4, 32, 32) == 1
+    assert L.ngapnet_wgrad_parts(256, 256, 256, 32, 32) == 256
+
+
 def test_strerror_and_defaults_without_gpu():
     assert _lib.strerror(0) == "ok"
     assert "setup" in _lib.strerror(_lib.ERR_NOT_SETUP)
@@ -123,3 +138,12 @@ def test_divergence_of_gradient_is_operator():
     o = Oracle.from_cfg(cfg, P.opr)
     Ap = o.apply_exact(interior(p, cfg))
     assert np.allclose(np.transpose(lhs, (2, 1, 0)).ravel(), Ap, rtol=1e-10, atol=1e-12 * np.abs(Ap).max())
+
+
+def test_rhs_sequence_torch_matches_numpy():
+    """RhsSequence.rhs_device (torch FFT) restates rhs (numpy FFT); run on the CPU here."""
+    seq = inputs.RhsSequence(n=32, seed=7, kmax=8)
+    for step in (0, 5):
+        ref = seq.rhs(step).ravel(order="F")
+        got = seq.rhs_device(step, "cpu").numpy()
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()

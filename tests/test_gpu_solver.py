@@ -171,7 +171,7 @@ def test_device_entry_points_match_host():
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("which", ["channel_spectral", "spheres_two_phase"])
 def test_full_size_256(which):
-    """BASELINE sizes: FP64 plain and mixed both reach 1e-10 true residual and
+    """256^3 channel and two-phase spheres: FP64 plain and mixed both reach 1e-10 true residual and
     agree with the FP64 oracle to 1e-8 (mean-free relative L2)."""
     print(f"\n[full size {which}]", flush=True)
     P = inputs.WORKLOADS[which](n=256)
