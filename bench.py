@@ -6,8 +6,16 @@ guess, with opr, rhs and sol already resident in HBM (device entry points of
 libngapsolve.so). The headline mode is the FP64 flexible PCG with the FP32
 V-cycle; the plain FP64 method is timed beside it. Prints one JSON line.
 
+Outside the timed region the line also carries:
+- the 256^3 two-phase spheres solve (density ratio 1000), setup and solve,
+  mixed against plain FP64;
+- the learned-guess sequence (128^3 periodic, 200 correlated right-hand
+  sides): train on steps 0-149, then iterations and wall time on steps
+  150-199 against the previous-step guess (nga2_amd.learned).
+
   python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
                   [--no-cpu] [--cpu-threads T] [--roofline-reps R]
+                  [--no-two-phase] [--no-sequence] [--seq-n 128] [--seq-m 8] [--seq-epochs 6]
 """
 from __future__ import annotations
 
@@ -42,6 +50,11 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--no-two-phase", action="store_true", help="skip the 256^3 two-phase spheres leg")
+    ap.add_argument("--no-sequence", action="store_true", help="skip the learned-guess sequence leg")
+    ap.add_argument("--seq-n", type=int, default=128)
+    ap.add_argument("--seq-m", type=int, default=8)
+    ap.add_argument("--seq-epochs", type=int, default=6)
     ap.add_argument("--kernel-only", choices=["rbgs", "spmv", "restrict", "prolong"], default=None,
                     help="time one fine-level kernel (for rocprofv3 summaries and counters) and exit")
     return ap.parse_args()
@@ -79,6 +92,55 @@ def kernel_only(args):
                       "reps": args.roofline_reps}), flush=True)
     ds.destroy()
     return 0
+
+
+def two_phase_leg(args, dev, solves=3):
+    """256^3 two-phase spheres (density ratio 1000): setup + solve, mixed vs FP64."""
+    import torch
+
+    from nga2_amd import inputs
+    from nga2_amd._lib import FP64, MIXED
+    from nga2_amd.linsol import DeviceSolve
+
+    t0 = time.time()
+    P = inputs.spheres_two_phase(n=args.n)
+    log(f"[bench] generated {P.name} in {time.time() - t0:.1f}s")
+    cfg = P.cfg
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
+    d_sol = torch.zeros_like(d_rhs)
+    res = {"workload": f"{P.name}: {cfg.nx}^3 periodic, 50 spheres, density ratio 1000, arithmetic face "
+                       f"densities, rcvg {args.rcvg:g}, zero guess"}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        ds = DeviceSolve(cfg)
+        ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=prec)
+        d_sol.zero_()
+        ds.solve(d_rhs, d_sol)  # warm-up
+        torch.cuda.synchronize()
+        setup_ms = []
+        solve_ms = []
+        its = []
+        for _ in range(solves):
+            t = time.perf_counter()
+            ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=prec)
+            torch.cuda.synchronize()
+            setup_ms.append((time.perf_counter() - t) * 1e3)
+            d_sol.zero_()
+            t = time.perf_counter()
+            it, _ = ds.solve(d_rhs, d_sol)
+            torch.cuda.synchronize()
+            solve_ms.append((time.perf_counter() - t) * 1e3)
+            its.append(it)
+        st = ds.stats()
+        res[name] = {"iters": its[0], "solve_ms": round(float(np.median(solve_ms)), 3),
+                     "setup_ms": round(float(np.median(setup_ms)), 3), "true_rerr": st["true_rerr"],
+                     "fallback": st["fallback"]}
+        ds.destroy()
+        log(f"[bench] two-phase {name}: it={its[0]} solve {res[name]['solve_ms']} ms setup {res[name]['setup_ms']} ms")
+    res["speedup_solve_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
+    res["speedup_setup_solve_mixed_vs_fp64"] = round(
+        (res["fp64"]["solve_ms"] + res["fp64"]["setup_ms"]) / (res["mixed"]["solve_ms"] + res["mixed"]["setup_ms"]), 3)
+    return res
 
 
 def main():
@@ -183,6 +245,21 @@ def main():
             err_fp64 = err(fp64["sol"])
         o.close()
 
+    two_phase = None
+    if not args.no_two_phase:
+        two_phase = two_phase_leg(args, dev)
+    sequence = None
+    if not args.no_sequence:
+        from nga2_amd import learned
+
+        log(f"[bench] learned-guess sequence ({args.seq_n}^3, 200 steps, train on 0-149)")
+        sequence = learned.run(n=args.seq_n, m=args.seq_m, epochs=args.seq_epochs, log=log)
+        ev = sequence["eval"]
+        sequence["workload"] = (f"{args.seq_n}^3 periodic, 200 correlated RHS (k<=16, per-mode phase speeds), "
+                                f"mixed precision, rcvg {args.rcvg:g}; steps 0-149 train, 150-199 evaluate")
+        sequence["speedup_learned_vs_prev"] = round(ev["ms_prev"] / ev["ms_learned"], 3)
+        sequence["speedup_extrap_vs_prev"] = round(ev["ms_prev"] / ev["ms_extrap"], 3)
+
     steps = args.steps
     el = mixed["elapsed"]
     value = steps / el
@@ -204,6 +281,10 @@ def main():
                              "err_vs_fp64_oracle": err_fp64,
                              "ms_per_solve": round(1e3 * fp64["elapsed"] / len(fp64["its"]), 3)}
         out["speedup_mixed_vs_fp64"] = round(out["value"] / out["fp64_plain"]["value"], 3)
+    if two_phase:
+        out["two_phase_256"] = two_phase
+    if sequence:
+        out["learned_sequence"] = sequence
     print(json.dumps(out), flush=True)
     return 0
 

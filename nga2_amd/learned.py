@@ -1,27 +1,38 @@
 """Learned initial guess over a sequence of pressure solves.
 
 The workload is the correlated right-hand-side sequence of inputs.RhsSequence:
-a 128^3 periodic box with 200 steps.
+a 128^3 periodic box with 200 steps. Training uses only what the solver
+itself produces on the training steps. The guess for step n has two learned
+parts.
 
-1. ``collect``: solve steps 0..T-1, each from the previous step's solution
-   (the method without the network). For each step it records
-   r_n = b_n - A p_{n-1} and e_n = p_n - p_{n-1}, both over the interior
-   cells with x fastest.
-2. ``train``: fit the U-Net (unet.UNet, trained by the project's own HIP
-   forward and backward kernels) so that kappa s_n N(r_n / s_n) leaves the
-   smallest residual r_n - A kappa s_n N(r_n / s_n). Here s_n = rms(r_n), and
-   kappa is the median of rms(e_n)/rms(r_n) over the training pairs. The
-   alternative loss fits N(r_n / s_n) to e_n / (kappa s_n). The periodic
-   problem fixes p only up to a constant, so the mean of e_n is removed.
-3. ``evaluate``: solve each of steps T..199 twice from the same p_{n-1}, to
-   the same tolerance with the same stopping rule:
-   - from the guess p_{n-1} (previous-step guess);
-   - with the solver's guess stage attached (ngap_set_stage), which starts
-     from p_{n-1} + kappa s N(r / s) and keeps that guess only if it lowers
-     the residual.
-   It reports iterations, solve wall time (the stage's own cost included) and
-   the true residuals. With ``precond_iters`` > 0 the network also acts as a
-   flexible-PCG stage in the first iterations.
+- Temporal part: a linear predictor of the solution increment from the m
+  previous increments, x_ext = p_{n-1} + sum_j alpha_j (p_{n-j} - p_{n-j-1}).
+  The m coefficients are the least-squares fit over the training solutions.
+- Spatial part: the U-Net (unet.UNet, trained by the project's own HIP
+  forward and backward kernels). It maps the residual r' = b_n - A x_ext of
+  the extrapolated guess to a correction e' ~ A^{-1} r'. It is applied
+  scale-free (kappa s N(r' / s), s = rms(r')) by the solver's guess stage
+  (ngap_set_stage), which keeps the corrected guess only if it lowers the
+  residual.
+
+Steps:
+
+1. ``collect``: solve steps 0..T-1, each from the previous solution (the
+   method without the network), and keep the solutions.
+2. ``fit_predictor``: fit the m coefficients alpha from the training
+   solutions.
+3. ``stage_pairs`` + ``train``: build the pairs (r', e' = p_n - x_ext) on the
+   training steps and fit the network to them. The default loss is the
+   residual the correction leaves, ||r'/s - kappa A N(r'/s)||^2. Here
+   kappa is the median of rms(e')/rms(r'). The periodic problem fixes p only
+   up to a constant, so increments and corrections are mean-free.
+4. ``evaluate``: solve each of steps T..199 three times from the same
+   history, to the same tolerance with the same stopping rule:
+   - from the previous solution (previous-step guess);
+   - from x_ext (temporal part alone);
+   - from x_ext with the network stage (the learned guess).
+   It reports iterations, solve wall time (the stage's own cost included),
+   initial and true residuals.
 """
 from __future__ import annotations
 
@@ -40,11 +51,19 @@ def interior_flat(t, cfg):
     return t.reshape(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx].reshape(-1)
 
 
-def add_interior(t, e, cfg):
-    """t[interior] += e in place (t ghosted Fortran-ordered flat, e interior x fastest)."""
+def to_ghosted(e, cfg):
+    """Ghosted Fortran-ordered flat array holding the interior vector e (ghosts 0)."""
+    import torch
+
+    g = torch.zeros(cfg.nxo * cfg.nyo * cfg.nzo, dtype=e.dtype, device=e.device)
     no = cfg.no
-    v = t.reshape(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx]
-    v += e.reshape(cfg.nz, cfg.ny, cfg.nx)
+    g.reshape(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx] = \
+        e.reshape(cfg.nz, cfg.ny, cfg.nx)
+    return g
+
+
+def _mean_free(v):
+    return v - v.mean()
 
 
 class SequenceSolver:
@@ -66,46 +85,89 @@ class SequenceSolver:
     def rhs(self, step):
         return self.seq.rhs_device(step, self.dev)
 
-    def residual(self, b, x):
-        """(r = b - A x over the interior, ||r||_2)"""
-        nrm = self.ds.residual(b, x, self._res)
+    def residual(self, b, x_int):
+        """(r = b - A x over the interior, ||r||_2) for an interior guess x_int."""
+        nrm = self.ds.residual(b, to_ghosted(x_int, self.cfg), self._res)
         return self._res.clone(), nrm
 
-    def solve(self, b, x0, stage=None, flags=_lib.STAGE_GUESS, precond_iters=0):
-        """Solve from the guess x0 (not modified); returns (x, stats)."""
+    def solve(self, b, x_int, stage=None, flags=_lib.STAGE_GUESS, precond_iters=0):
+        """Solve from the interior guess x_int; returns (interior solution, stats)."""
         import torch
 
-        x = x0.clone()
+        x = to_ghosted(x_int, self.cfg)
         self.ds.set_stage(stage, flags=flags, precond_iters=precond_iters)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        self.ds.solve(b, x)
-        wall = (time.perf_counter() - t0) * 1e3
-        st = self.ds.stats()
+        try:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            self.ds.solve(b, x)
+            wall = (time.perf_counter() - t0) * 1e3
+            st = self.ds.stats()
+        finally:
+            self.ds.set_stage(None)
         st["wall_ms"] = wall
-        self.ds.set_stage(None)
-        return x, st
+        return interior_flat(x, self.cfg).clone(), st
 
     def close(self):
         self.ds.destroy()
 
 
 def collect(S, steps, x0=None):
-    """Solve `steps` from the previous solution; return the pairs and the
-    last solution. Pairs are (r_n, e_n) FP64 device vectors."""
+    """Solve `steps` in order, each from the previous solution. Returns the
+    mean-free interior solutions and the solve statistics."""
     import torch
 
-    x = x0 if x0 is not None else torch.zeros(S.cfg.nxo * S.cfg.nyo * S.cfg.nzo, dtype=torch.float64, device=S.dev)
-    pairs, stats = [], []
+    x = x0 if x0 is not None else torch.zeros(S.N, dtype=torch.float64, device=S.dev)
+    sols, stats = [], []
     for n in steps:
-        b = S.rhs(n)
-        r, _ = S.residual(b, x)
-        xn, st = S.solve(b, x)
-        e = interior_flat(xn, S.cfg) - interior_flat(x, S.cfg)
-        pairs.append((r, e - e.mean()))
+        xn, st = S.solve(S.rhs(n), x)
+        x = _mean_free(xn)
+        sols.append(x)
         stats.append(st)
-        x = xn
-    return pairs, stats, x
+    return sols, stats
+
+
+def fit_predictor(sols, m):
+    """alpha minimising sum_n ||d_n - sum_j alpha_j d_{n-j}||^2 over the
+    increments d_n = p_n - p_{n-1} of the training solutions (n >= m + 1).
+    Solved from the m x m normal equations in float64, with a 1e-14 relative
+    ridge."""
+    import torch
+
+    d = [sols[i] - sols[i - 1] for i in range(1, len(sols))]
+    if len(d) <= m:
+        raise ValueError(f"need more than {m + 1} training solutions")
+    G = np.zeros((m, m))
+    h = np.zeros(m)
+    for t in range(m, len(d)):
+        past = [d[t - j] for j in range(1, m + 1)]
+        for i in range(m):
+            h[i] += float(torch.dot(past[i], d[t]))
+            for j in range(i, m):
+                G[i, j] += float(torch.dot(past[i], past[j]))
+    G = G + np.triu(G, 1).T
+    G += 1e-14 * np.trace(G) / m * np.eye(m)
+    return np.linalg.solve(G, h)
+
+
+def extrapolate(hist, alpha):
+    """x_ext from the last m + 1 solutions (oldest first)."""
+    m = len(alpha)
+    x = hist[-1].clone()
+    for j in range(1, m + 1):
+        x += float(alpha[j - 1]) * (hist[-j] - hist[-j - 1])
+    return x
+
+
+def stage_pairs(S, sols, alpha, steps):
+    """(r', e') pairs of the network stage for the given training steps
+    (indices into sols, each >= m + 1)."""
+    m = len(alpha)
+    pairs = []
+    for n in steps:
+        x_ext = extrapolate(sols[n - m - 1:n], alpha)
+        r, _ = S.residual(S.rhs(n), x_ext)
+        pairs.append((r, _mean_free(sols[n] - x_ext)))
+    return pairs
 
 
 def fit_kappa(pairs):
@@ -166,71 +228,79 @@ def train(net, pairs, kappa, S=None, epochs=8, lr=2e-3, lr_end=1e-4, seed=0, los
     return hist
 
 
-def evaluate(S, net, steps, x0, precond_iters=0):
-    """Solve each step from p_{n-1} with and without the guess stage.
-    Returns per-step records."""
+def evaluate(S, net, alpha, sols, steps, precond_iters=0):
+    """Solve each step from the same history three ways (previous solution,
+    temporal extrapolation, extrapolation + network stage). `sols` holds at
+    least the last m + 1 solutions before the first step; the history is
+    advanced with the previous-step solve's solution. Returns per-step
+    records."""
     import torch
 
+    m = len(alpha)
+    hist = list(sols[-(m + 1):])
+    stage = net.stage() if net is not None else None
+    flags = _lib.STAGE_GUESS | (_lib.STAGE_PRECOND if precond_iters > 0 else 0)
     out = []
-    x = x0
-    stage = net.stage()
     for n in steps:
         b = S.rhs(n)
-        r, rn = S.residual(b, x)
         bn = float(torch.linalg.vector_norm(interior_flat(b, S.cfg)))
-        xa, sa = S.solve(b, x)
-        flags = _lib.STAGE_GUESS | (_lib.STAGE_PRECOND if precond_iters > 0 else 0)
-        xb, sb = S.solve(b, x, stage=stage, flags=flags, precond_iters=precond_iters)
-        # residual of the learned guess alone (what the stage starts from)
-        g = x.clone()
-        add_interior(g, net.apply(r), S.cfg)
-        _, rg = S.residual(b, g)
-        dx = interior_flat(xb, S.cfg) - interior_flat(xa, S.cfg)
-        dx = dx - dx.mean()
-        xa_i = interior_flat(xa, S.cfg)
-        xa_i = xa_i - xa_i.mean()
-        out.append({
-            "step": n,
-            "r0_prev": rn / bn, "r0_learned": rg / bn,
-            "it_prev": sa["it"], "it_learned": sb["it"],
-            "ms_prev": sa["solve_ms"], "ms_learned": sb["solve_ms"],
-            "true_prev": sa["true_rerr"], "true_learned": sb["true_rerr"],
-            "stage_calls": sb["stage_calls"], "fallback": sb["fallback"],
-            "diff": float(torch.linalg.vector_norm(dx) / torch.linalg.vector_norm(xa_i)),
-        })
-        x = xa
-    return out, x
+        x_prev = hist[-1]
+        x_ext = extrapolate(hist, alpha)
+        _, r_prev = S.residual(b, x_prev)
+        r_ext_v, r_ext = S.residual(b, x_ext)
+        xa, sa = S.solve(b, x_prev)
+        xb, sb = S.solve(b, x_ext)
+        rec = {"step": n, "r0_prev": r_prev / bn, "r0_extrap": r_ext / bn,
+               "it_prev": sa["it"], "it_extrap": sb["it"],
+               "ms_prev": sa["solve_ms"], "ms_extrap": sb["solve_ms"],
+               "true_prev": sa["true_rerr"], "true_extrap": sb["true_rerr"]}
+        ref = _mean_free(xa)
+        nref = float(torch.linalg.vector_norm(ref))
+        diffs = [float(torch.linalg.vector_norm(_mean_free(xb) - ref)) / nref]
+        if net is not None:
+            xc, sc = S.solve(b, x_ext, stage=stage, flags=flags, precond_iters=precond_iters)
+            _, r_learn = S.residual(b, x_ext + net.apply(r_ext_v))
+            rec.update({"r0_learned": r_learn / bn, "it_learned": sc["it"], "ms_learned": sc["solve_ms"],
+                        "true_learned": sc["true_rerr"], "stage_calls": sc["stage_calls"],
+                        "rejected": int(bool(sc["fallback"] & 2))})
+            diffs.append(float(torch.linalg.vector_norm(_mean_free(xc) - ref)) / nref)
+        rec["diff"] = max(diffs)
+        out.append(rec)
+        hist = hist[1:] + [_mean_free(xa)]
+    return out
 
 
 def summary(recs):
-    f = lambda k: float(np.mean([r[k] for r in recs]))  # noqa: E731
-    return {
-        "steps": len(recs),
-        "it_prev": f("it_prev"), "it_learned": f("it_learned"),
-        "ms_prev": f("ms_prev"), "ms_learned": f("ms_learned"),
-        "r0_prev": f("r0_prev"), "r0_learned": f("r0_learned"),
-        "max_true_prev": max(r["true_prev"] for r in recs),
-        "max_true_learned": max(r["true_learned"] for r in recs),
-        "max_solution_diff": max(r["diff"] for r in recs),
-        "guess_rejected": int(sum(1 for r in recs if r["fallback"] & 2)),
-    }
+    keys = [k for k in recs[0] if k.startswith(("it_", "ms_", "r0_"))]
+    s = {"steps": len(recs)}
+    for k in keys:
+        s[k] = float(np.mean([r[k] for r in recs]))
+    for k in [k for k in recs[0] if k.startswith("true_")]:
+        s["max_" + k] = max(r[k] for r in recs)
+    s["max_solution_diff"] = max(r["diff"] for r in recs)
+    if "rejected" in recs[0]:
+        s["guess_rejected"] = int(sum(r["rejected"] for r in recs))
+    return s
 
 
-def run(n=128, steps=200, train_steps=150, epochs=8, lr=2e-3, seed=0, precond_iters=0, log=print):
+def run(n=128, steps=200, train_steps=150, m=6, epochs=6, lr=5e-4, seed=0, precond_iters=0, log=print):
     """The whole workflow; returns a dict of timings and summaries."""
     t0 = time.perf_counter()
     S = SequenceSolver(n=n)
-    pairs, st_train, x = collect(S, range(train_steps))
+    sols, st_train = collect(S, range(train_steps))
     t_collect = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    alpha = fit_predictor(sols, m)
+    pairs = stage_pairs(S, sols, alpha, range(m + 1, train_steps))
     kappa = fit_kappa(pairs)
     net = UNet((n, n, n), seed=seed)
-    t1 = time.perf_counter()
     hist = train(net, pairs, kappa, S=S, epochs=epochs, lr=lr, log=log)
     t_train = time.perf_counter() - t1
     del pairs
-    recs, _ = evaluate(S, net, range(train_steps, steps), x, precond_iters=precond_iters)
-    res = {"kappa": kappa, "loss_history": hist, "collect_s": t_collect, "train_s": t_train,
-           "train_solves_it": float(np.mean([s["it"] for s in st_train])), "eval": summary(recs)}
+    recs = evaluate(S, net, alpha, sols, range(train_steps, steps), precond_iters=precond_iters)
+    res = {"alpha": [float(a) for a in alpha], "kappa": kappa, "loss_history": hist, "collect_s": t_collect,
+           "train_s": t_train, "train_solves_it": float(np.mean([s["it"] for s in st_train])),
+           "eval": summary(recs)}
     net.close()
     S.close()
     return res

@@ -147,3 +147,20 @@ def test_rhs_sequence_torch_matches_numpy():
         ref = seq.rhs(step).ravel(order="F")
         got = seq.rhs_device(step, "cpu").numpy()
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_temporal_predictor_recovers_a_rotation():
+    """Increments of p_n = cos(w n) v + sin(w n) u obey d_n = 2 cos(w) d_{n-1} - d_{n-2}:
+    the least-squares fit with m = 2 returns (2 cos w, -1) and extrapolates exactly."""
+    import torch
+    from nga2_amd import learned
+
+    g = torch.Generator().manual_seed(0)
+    v, u = torch.randn(500, generator=g, dtype=torch.float64), torch.randn(500, generator=g, dtype=torch.float64)
+    w = 0.3
+    sols = [np.cos(w * n) * v + np.sin(w * n) * u for n in range(30)]
+    alpha = learned.fit_predictor(sols, 2)
+    assert np.allclose(alpha, [2 * np.cos(w), -1.0], rtol=0, atol=1e-10)
+    x = learned.extrapolate(sols[-3:], alpha)
+    exact = np.cos(w * 30) * v + np.sin(w * 30) * u
+    assert float(torch.max(torch.abs(x - exact))) <= 1e-9
