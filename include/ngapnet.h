@@ -20,8 +20,11 @@ int ngapnet_conv3_wgrad(const void *in_bf16, const void *dout_bf16, float *dw, f
 int ngapnet_flip_transpose(const void *wt_bf16, void *wd_bf16, int cin, int cout, void *stream);
 int ngapnet_conv3_small_fwd(const float *in, const float *w, float *out32, void *outb_bf16, int nx, int ny, int nz,
                             int cin, int cout, int relu, void *stream);
-int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, int nx, int ny, int nz, int cin, int cout,
-                              void *stream);
+/* part: NULL, or ngapnet_small_wgrad_part_size(cout) floats of scratch for the
+ * one-pass kernel (cin == 1, cout == 32); other shapes use one workgroup per weight */
+int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, float *part, int nx, int ny, int nz,
+                              int cin, int cout, void *stream);
+long long ngapnet_small_wgrad_part_size(int cout);
 int ngapnet_relu_bwd(const float *dy, const void *y_bf16, float *dx, void *dxb_bf16, long long n, void *stream);
 int ngapnet_to_bf16(const float *s, void *d_bf16, long long n, void *stream);
 int ngapnet_pool2(const void *in_bf16, void *out_bf16, int nx, int ny, int nz, int c, void *stream);

@@ -191,6 +191,16 @@ __global__ void __launch_bounds__(NT) k_dot(long long n, const Tx *__restrict__ 
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// The PCG vector kernels walk the vectors in pairs: one 16-byte load per FP64
+// array (8 bytes for an FP32 one) per thread and step. An odd last element is
+// taken by thread 0 of block 0.
+struct alignas(16) D2 {
+    double a, b;
+};
+struct alignas(8) F2 {
+    float a, b;
+};
+
 // x += alpha p ; r -= alpha q ; partial ||r||^2 ; optional FP32 copy of r
 __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restrict__ x, double *__restrict__ r,
                                                   const double *__restrict__ p, const double *__restrict__ q,
@@ -199,7 +209,25 @@ __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restric
     __shared__ double sm[NT / 64];
     const double a = scal[S_ALPHA];
     double acc = 0.0;
-    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
+    const long long n2 = n >> 1;
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n2; i += (long long)gridDim.x * NT) {
+        D2 rv = reinterpret_cast<const D2 *>(r)[i];
+        if (a != 0.0) {
+            D2 xv = reinterpret_cast<const D2 *>(x)[i];
+            const D2 pv = reinterpret_cast<const D2 *>(p)[i], qv = reinterpret_cast<const D2 *>(q)[i];
+            xv.a += a * pv.a;
+            xv.b += a * pv.b;
+            rv.a -= a * qv.a;
+            rv.b -= a * qv.b;
+            reinterpret_cast<D2 *>(x)[i] = xv;
+            reinterpret_cast<D2 *>(r)[i] = rv;
+        }
+        acc += rv.a * rv.a;
+        acc += rv.b * rv.b;
+        if (r32) reinterpret_cast<F2 *>(r32)[i] = F2{(float)rv.a, (float)rv.b};
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long i = n - 1;
         double rv = r[i];
         if (a != 0.0) {
             x[i] += a * p[i];
@@ -213,15 +241,41 @@ __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restric
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+template <typename Tz>
+struct Pair;
+template <>
+struct Pair<double> {
+    typedef D2 type;
+};
+template <>
+struct Pair<float> {
+    typedef F2 type;
+};
+
 // <r,z> and (optionally) <z,q>
 template <typename Tz>
 __global__ void __launch_bounds__(NT) k_dot_rz(long long n, const double *__restrict__ r, const Tz *__restrict__ z,
                                                const double *__restrict__ q, double *__restrict__ p_rz,
                                                double *__restrict__ p_zq) {
+    typedef typename Pair<Tz>::type P;
     __shared__ double sm[NT / 64];
     double a = 0.0, b = 0.0;
-    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
-        double zv = (double)z[i];
+    const long long n2 = n >> 1;
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n2; i += (long long)gridDim.x * NT) {
+        const P zv = reinterpret_cast<const P *>(z)[i];
+        const D2 rv = reinterpret_cast<const D2 *>(r)[i];
+        const double z0 = (double)zv.a, z1 = (double)zv.b;
+        a += rv.a * z0;
+        a += rv.b * z1;
+        if (q) {
+            const D2 qv = reinterpret_cast<const D2 *>(q)[i];
+            b += z0 * qv.a;
+            b += z1 * qv.b;
+        }
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long i = n - 1;
+        const double zv = (double)z[i];
         a += r[i] * zv;
         if (q) b += zv * q[i];
     }
@@ -236,9 +290,25 @@ __global__ void __launch_bounds__(NT) k_dot_rz(long long n, const double *__rest
 template <typename Tz>
 __global__ void __launch_bounds__(NT) k_update_p(long long n, double *__restrict__ p, const Tz *__restrict__ z,
                                                  const double *__restrict__ scal, int copy) {
+    typedef typename Pair<Tz>::type P;
     const double be = copy ? 0.0 : scal[S_BETA];
-    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += (long long)gridDim.x * NT) {
-        double zv = (double)z[i];
+    const long long n2 = n >> 1;
+    for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n2; i += (long long)gridDim.x * NT) {
+        const P zv = reinterpret_cast<const P *>(z)[i];
+        D2 pv;
+        if (copy) {
+            pv.a = (double)zv.a;
+            pv.b = (double)zv.b;
+        } else {
+            pv = reinterpret_cast<const D2 *>(p)[i];
+            pv.a = (double)zv.a + be * pv.a;
+            pv.b = (double)zv.b + be * pv.b;
+        }
+        reinterpret_cast<D2 *>(p)[i] = pv;
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+        const long long i = n - 1;
+        const double zv = (double)z[i];
         p[i] = copy ? zv : zv + be * p[i];
     }
 }
@@ -648,17 +718,46 @@ void coarse_solve(ngap_t *h, const T *f, T *v) {
     }
 }
 
+// First level of the small-level bottom (k_bottom), or nlev when the
+// bottom is not used: it needs the dense coarsest solve, at least one
+// smoothing level below the fine level, and at most BOT_MAX_LEVELS levels.
+int bottom_start(const ngap_t *h) {
+    const int nl = h->nlev;
+    if (!h->coarse_dense || nl < 3) return nl;
+    int lb = nl - 1;
+    while (lb - 1 >= 1 && h->lev[lb - 1].g.N <= BOT_MAX_CELLS && nl - (lb - 1) <= BOT_MAX_LEVELS) lb--;
+    return lb <= nl - 2 ? lb : nl;
+}
+
+template <typename T>
+void launch_bottom_levels(ngap_t *h, int lb) {
+    BotArgs<T> a{};
+    a.nlev = h->nlev - lb;
+    a.npre = h->opts.nu_pre;
+    a.npost = h->opts.nu_post;
+    a.Ainv = h->Ainv;
+    for (int q = 0; q < a.nlev; q++) {
+        Level &L = h->lev[lb + q];
+        LevelData<T> &D = data<T>(L);
+        a.L[q] = BotLevel<T>{L.g, L.cdir, D.c, D.a0, D.a1, D.a2, D.pm, D.pp, D.f, D.v, D.w};
+    }
+    launch_bottom<T>(h->stream, a);
+}
+
 // One V(nu_pre, nu_post) cycle from a zero guess: v0 = M f0.
 // Sweeps are out of place; on each level the iterate alternates between the
 // level's two buffers so that the last post-smoothing sweep lands in the
-// final buffer (v0 on the fine level, D.v on coarse levels).
+// final buffer (v0 on the fine level, D.v on coarse levels). The levels from
+// bottom_start() down run inside one workgroup (k_bottom), result in D.v.
 template <typename T>
 void vcycle(ngap_t *h, const T *f0, T *v0) {
     const int nl = h->nlev;
     hipStream_t s = h->stream;
     const int npre = h->opts.nu_pre, npost = h->opts.nu_post;
+    const int lb = bottom_start(h);
+    const int top = lb < nl ? lb : nl - 1;  // first level below the per-level loop
     std::vector<T *> pre(nl);
-    for (int l = 0; l < nl - 1; l++) {
+    for (int l = 0; l < top; l++) {
         Level &L = h->lev[l];
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
@@ -682,12 +781,14 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         launch_resid<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, rbuf);
         launch_gather<T>(s, L.g, Cn.g, L.cdir, rbuf, DC.pm, DC.pp, DC.f);
     }
-    {
+    if (lb < nl) {
+        launch_bottom_levels<T>(h, lb);
+    } else {
         Level &C = h->lev[nl - 1];
         LevelData<T> &D = data<T>(C);
         coarse_solve<T>(h, nl == 1 ? f0 : D.f, nl == 1 ? v0 : D.v);
     }
-    for (int l = nl - 2; l >= 0; l--) {
+    for (int l = top - 1; l >= 0; l--) {
         Level &L = h->lev[l];
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
@@ -703,7 +804,6 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         }
     }
 }
-
 
 // ---------------------------------------------------------------- graphs
 

@@ -36,6 +36,27 @@ void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T
 template <typename Tin, typename Tout>
 void launch_dense(hipStream_t s, int n, const double *Ainv, const Tin *f, Tout *v);
 int apply_blocks(const Geo &g, int &kc);
+
+// The small levels of a V-cycle in one workgroup (see k_bottom).
+constexpr int BOT_MAX_LEVELS = 16;
+constexpr long long BOT_MAX_CELLS = 4096;
+template <typename T>
+struct BotLevel {
+    Geo g;
+    int cdir;  // coarsening direction to the next level
+    const T *c, *a0, *a1, *a2;
+    const T *pm, *pp;  // transfer weights from the finer level (coarse-indexed)
+    T *f, *v, *w;
+};
+template <typename T>
+struct BotArgs {
+    int nlev;  // levels lb..lb+nlev-1, the last one solved by the dense inverse
+    int npre, npost;
+    const double *Ainv;
+    BotLevel<T> L[BOT_MAX_LEVELS];
+};
+template <typename T>
+void launch_bottom(hipStream_t s, const BotArgs<T> &a);
 void launch_coarse_build(hipStream_t s, const Geo &F, const Geo &C, int d, const double *c, const double *a0,
                          const double *a1, const double *a2, double *pm, double *pp, double *cc, double *ca0,
                          double *ca1, double *ca2, double *sc);

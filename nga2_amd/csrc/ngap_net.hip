@@ -245,6 +245,101 @@ __global__ void __launch_bounds__(256) k_conv3_small_wgrad(const float *__restri
     if (threadIdx.x == 0) dw[wi] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
+// dW of a 1-input-channel layer, one pass per row of taps: workgroup (b, row)
+// with row = (dz+1)*3 + (dy+1) accumulates, for its voxels, the three taps
+// dx = -1, 0, 1 of that row for all COUT channels (3*COUT registers), reading
+// each voxel's dout row once as COUT/4 16-byte loads. Partials per workgroup,
+// summed in fixed order by k_small_wgrad_fin.
+constexpr int SW_BLOCKS = 256;
+
+template <int COUT>
+__global__ void __launch_bounds__(256) k_small_wgrad_rows(const float *__restrict__ in, const float *__restrict__ dout,
+                                                          float *__restrict__ part, int nx, int ny, int nz) {
+    __shared__ float sm[4][3 * COUT];
+    const int row = blockIdx.y;
+    const int dz = row / 3 - 1, dy = row % 3 - 1;
+    const long long nv = (long long)nx * ny * nz;
+    float acc[3][COUT];
+#pragma unroll
+    for (int e = 0; e < 3; e++)
+#pragma unroll
+        for (int c = 0; c < COUT; c++) acc[e][c] = 0.f;
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        const long long base = (long long)nx * (wrapi(y + dy, ny) + (long long)ny * wrapi(z + dz, nz));
+        const float xm = in[base + wrapi(x - 1, nx)], x0 = in[base + x], xp = in[base + wrapi(x + 1, nx)];
+        const float4 *drow = reinterpret_cast<const float4 *>(dout + v * COUT);
+#pragma unroll
+        for (int q = 0; q < COUT / 4; q++) {
+            const float4 d = drow[q];
+            const float dd[4] = {d.x, d.y, d.z, d.w};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                acc[0][4 * q + k] += dd[k] * xm;
+                acc[1][4 * q + k] += dd[k] * x0;
+                acc[2][4 * q + k] += dd[k] * xp;
+            }
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int e = 0; e < 3; e++)
+#pragma unroll
+        for (int c = 0; c < COUT; c++) {
+            float t = acc[e][c];
+            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+            if (lane == 0) sm[wave][e * COUT + c] = t;
+        }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * COUT; i += 256)
+        part[((long long)blockIdx.x * 9 + row) * 3 * COUT + i] = sm[0][i] + sm[1][i] + sm[2][i] + sm[3][i];
+}
+
+// dw[tap][co] = sum_b part[b][tap / 3][tap % 3][co]
+__global__ void k_small_wgrad_fin(const float *__restrict__ part, float *__restrict__ dw, int nb, int cout) {
+    const int n = 27 * cout;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const int tap = i / cout, co = i % cout;
+        const int row = tap / 3, e = tap % 3;
+        float s = 0.f;
+        for (int b = 0; b < nb; b++) s += part[((long long)b * 9 + row) * 3 * cout + e * cout + co];
+        dw[i] = s;
+    }
+}
+
+// dx[v][c] = w[c] dy[v]; dW partials per block (C known at compile time, registers only)
+template <int C>
+__global__ void __launch_bounds__(256) k_conv1_bwd_t(const __bf16 *__restrict__ x, const float *__restrict__ w,
+                                                     const float *__restrict__ dy, float *__restrict__ dx,
+                                                     float *__restrict__ part, long long nv) {
+    __shared__ float sm[4][C];
+    float accw[C];
+    float wr[C];
+#pragma unroll
+    for (int ch = 0; ch < C; ch++) {
+        accw[ch] = 0.f;
+        wr[ch] = w[ch];
+    }
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
+        const float g = dy[v];
+#pragma unroll
+        for (int ch = 0; ch < C; ch++) {
+            dx[v * C + ch] = wr[ch] * g;
+            accw[ch] += g * (float)x[v * C + ch];
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int ch = 0; ch < C; ch++) {
+        float t = accw[ch];
+        for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+        if (lane == 0) sm[wave][ch] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < C)
+        part[blockIdx.x * C + threadIdx.x] = sm[0][threadIdx.x] + sm[1][threadIdx.x] + sm[2][threadIdx.x] + sm[3][threadIdx.x];
+}
+
 // ---------------------------------------------------------------- elementwise
 __global__ void k_relu_bwd(const float *__restrict__ dy, const __bf16 *__restrict__ y, float *__restrict__ dx,
                            __bf16 *__restrict__ dxb, long long n) {
@@ -446,12 +541,21 @@ int ngapnet_conv3_small_fwd(const float *in, const float *w, float *out32, void 
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, int nx, int ny, int nz, int cin, int cout,
-                              void *stream) {
-    hipLaunchKernelGGL(k_conv3_small_wgrad, dim3(27 * cin * cout), dim3(256), 0, (hipStream_t)stream, in, dout, dw,
-                       nx, ny, nz, cin, cout);
+int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, float *part, int nx, int ny, int nz,
+                              int cin, int cout, void *stream) {
+    if (part && cin == 1 && cout == 32) {
+        hipLaunchKernelGGL(k_small_wgrad_rows<32>, dim3(SW_BLOCKS, 9), dim3(256), 0, (hipStream_t)stream, in, dout, part,
+                           nx, ny, nz);
+        hipLaunchKernelGGL(k_small_wgrad_fin, dim3(blocks_for(27LL * cout)), dim3(256), 0, (hipStream_t)stream,
+                           (const float *)part, dw, SW_BLOCKS, cout);
+    } else {
+        hipLaunchKernelGGL(k_conv3_small_wgrad, dim3(27 * cin * cout), dim3(256), 0, (hipStream_t)stream, in, dout, dw,
+                           nx, ny, nz, cin, cout);
+    }
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
+
+long long ngapnet_small_wgrad_part_size(int cout) { return (long long)SW_BLOCKS * 27 * cout; }
 
 int ngapnet_relu_bwd(const float *dy, const void *y, float *dx, void *dxb, long long n, void *stream) {
     hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, dy, (const __bf16 *)y, dx,
@@ -512,8 +616,12 @@ int ngapnet_conv1_bwd(const void *x, const float *w, const float *dy, float *dx,
                       int c, void *stream) {
     if (c > 64) return -1;
     const int nb = 1024;
-    hipLaunchKernelGGL(k_conv1_bwd, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, dy, dx, part,
-                       nv, c);
+    if (c == 32)
+        hipLaunchKernelGGL(k_conv1_bwd_t<32>, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, dy, dx,
+                           part, nv);
+    else
+        hipLaunchKernelGGL(k_conv1_bwd, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, dy, dx,
+                           part, nv, c);
     hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, (hipStream_t)stream, (const float *)part, dw, nb, c);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }

@@ -154,7 +154,7 @@ int alloc_grads(ngapnet *M) {
     long long big = V[0] * 64;
     if (V[1] * 96 > big) big = V[1] * 96;
     if (V[2] * 128 > big) big = V[2] * 128;
-    long long parts = 1;
+    long long parts = ngapnet_small_wgrad_part_size(32);
     for (int i = 0; i < NGAPNET_NLAYERS; i++)
         if (kSpec[i].kind == MFMA) {
             const int *s = M->shp[kSpec[i].lvl];
@@ -244,7 +244,7 @@ int backward(ngapnet *M, const float *dy, hipStream_t s) {
     NCK(conv_bwd(M, C1, M->p1, gb, fa, s));  // fa = d(p1)
     NCK(ngapnet_pool2_bwd(fa, M->sk0, S[0][0], S[0][1], S[0][2], 32, 1, s));
     NCK(ngapnet_relu_bwd(M->sk0, M->c0, fc, nullptr, V[0] * 32, s));
-    NCK(ngapnet_conv3_small_wgrad(M->x, fc, M->g + M->off[C0], S[0][0], S[0][1], S[0][2], 1, 32, s));
+    NCK(ngapnet_conv3_small_wgrad(M->x, fc, M->g + M->off[C0], M->wparts, S[0][0], S[0][1], S[0][2], 1, 32, s));
     return 0;
 }
 

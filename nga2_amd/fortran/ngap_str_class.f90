@@ -19,6 +19,7 @@ module ngap_str_class
 
    public :: ngap_str
    public :: pfmg,pcg_pfmg,gmres_pfmg,smg,pcg_smg,gmres_smg,ngap_fp64,ngap_mixed
+   public :: ngap_stage_guess,ngap_stage_precond
 
    !> ngap_str object definition
    type, extends(linsol) :: ngap_str
@@ -34,6 +35,7 @@ module ngap_str_class
       procedure :: setup=>ngap_str_setup
       procedure :: solve=>ngap_str_solve
       procedure :: destroy=>ngap_str_destroy
+      procedure :: set_stage=>ngap_str_set_stage
    end type ngap_str
 
    interface ngap_str
@@ -134,6 +136,21 @@ contains
       if (verbose.gt.0) call this%log
       if (verbose.gt.1) call this%print_short
    end subroutine ngap_str_solve
+
+   !> Attach a correction stage (no counterpart in hypre_str): fn is c_funloc of
+   !> a bind(C) function with the signature of ngap_stage_fn (e.g. ngapnet_stage
+   !> of libngapnet.so) and ctx its context; flags ngap_stage_guess and/or
+   !> ngap_stage_precond. fn=c_null_funptr removes the stage. Call after init.
+   subroutine ngap_str_set_stage(this,fn,ctx,flags,precond_iters)
+      use messager, only: die
+      implicit none
+      class(ngap_str), intent(inout) :: this
+      type(c_funptr), intent(in) :: fn
+      type(c_ptr), intent(in) :: ctx
+      integer, intent(in) :: flags,precond_iters
+      if (.not.c_associated(this%handle)) call die('[ngap_str set_stage] Solver has not been initialised.')
+      call check(ngap_set_stage(this%handle,fn,ctx,int(flags,c_int),int(precond_iters,c_int)),'set_stage')
+   end subroutine ngap_str_set_stage
 
    !> Release the device data (mirrors hypre_str_destroy, :390-422)
    subroutine ngap_str_destroy(this)

@@ -53,7 +53,9 @@ def lib():
         L.ngapnet_conv3_wgrad.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]
         L.ngapnet_flip_transpose.argtypes = [_vp, _vp, _i, _i, _vp]
         L.ngapnet_conv3_small_fwd.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _vp]
-        L.ngapnet_conv3_small_wgrad.argtypes = [_vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]
+        L.ngapnet_conv3_small_wgrad.argtypes = [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _vp]
+        L.ngapnet_small_wgrad_part_size.argtypes = [_i]
+        L.ngapnet_small_wgrad_part_size.restype = _ll
         L.ngapnet_relu_bwd.argtypes = [_vp, _vp, _vp, _vp, _ll, _vp]
         L.ngapnet_to_bf16.argtypes = [_vp, _vp, _ll, _vp]
         L.ngapnet_pool2.argtypes = [_vp, _vp, _i, _i, _i, _i, _vp]

@@ -43,6 +43,10 @@ CASES = {
     "mixed12x9x8": lambda: _box(12, 9, 8, (True, False, True)),
     "thin16x16x1": lambda: _box(16, 16, 1, (True, True, True), walls=False),
     "rect40x6x18": lambda: _box(40, 6, 18, (True, False, False)),
+    # levels of 2^20 cells with rows of 128/256 cells: the row-wave kernels
+    # (one x-row per wavefront), with walls in y, and in x (row ends)
+    "rows256x16x256": lambda: _box(256, 16, 256, (True, False, True)),
+    "rows128x64x128": lambda: _box(128, 64, 128, (False, True, True)),
 }
 
 
@@ -124,7 +128,10 @@ def test_level_kernels_fp32(case):
         x = rng.standard_normal(n)
         f = rng.standard_normal(n)
         assert _close(hk.apply(l, x, prec=1), o.apply(l, x), 2e-5), ("apply", l)
-        assert _close(hk.rbgs(l, f, x, 0, prec=1), o.rbgs(l, f, x, 0), 1e-4), ("rbgs", l)
+        for order in (0, 1):
+            assert _close(hk.rbgs(l, f, x, order, prec=1), o.rbgs(l, f, x, order), 1e-4), ("rbgs", l, order)
+            zero = np.zeros(n)
+            assert _close(hk.rbgs(l, f, x, 2 | order, prec=1), o.rbgs(l, f, zero, order), 1e-4), ("rbgs0", l, order)
         if l + 1 < o.nlevels:
             assert _close(hk.resid_restrict(l, f, x, prec=1), o.resid_restrict(l, f, x), 2e-5), ("restrict", l)
     r = rng.standard_normal(o.N)

@@ -153,10 +153,13 @@ def test_small_conv_forward_and_weight_gradient(shape):
     assert np.abs(_np(out) - ref).max() <= 1e-5 * np.abs(ref).max()
     assert torch.equal(outb, torch.relu(out).to(torch.bfloat16))
     gy, gyr = _f32(rng.standard_normal((V, cout)))
-    dw = torch.zeros(27 * cout * cin, dtype=torch.float32, device=DEV)
-    _run(lambda L, p: L.ngapnet_conv3_small_wgrad(p(x), p(gy), p(dw), *shape, cin, cout, None))
     ref = _wgrad_direct(xr, gyr, shape, cin, cout)
-    assert np.abs(_np(dw) - ref).max() <= 1e-5 * np.abs(ref).max()
+    unet, L = _lib()
+    part = torch.zeros(int(L.ngapnet_small_wgrad_part_size(cout)), dtype=torch.float32, device=DEV)
+    for scratch in (None, part):  # one workgroup per weight / one pass per tap row
+        dw = torch.zeros(27 * cout * cin, dtype=torch.float32, device=DEV)
+        _run(lambda L, p: L.ngapnet_conv3_small_wgrad(p(x), p(gy), p(dw), p(scratch), *shape, cin, cout, None))
+        assert np.abs(_np(dw) - ref).max() <= 1e-5 * np.abs(ref).max()
 
 
 def test_pool_upsample_concat_relu_kernels():
