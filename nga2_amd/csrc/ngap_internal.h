@@ -39,7 +39,7 @@ int apply_blocks(const Geo &g, int &kc);
 
 // The small levels of a V-cycle in one workgroup (see k_bottom).
 constexpr int BOT_MAX_LEVELS = 16;
-constexpr long long BOT_MAX_CELLS = 4096;
+constexpr long long BOT_MAX_CELLS = 1024;
 template <typename T>
 struct BotLevel {
     Geo g;

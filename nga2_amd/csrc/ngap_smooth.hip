@@ -600,64 +600,118 @@ __global__ void __launch_bounds__(256) k_dense_t(int n, const double *__restrict
 // latency for a few thousand cells of work. Sweeps are in place (each colour
 // reads only the other colour, so this equals the out-of-place sweeps of the
 // large levels), and every sum keeps the order of the per-level kernels.
-// Global memory written before a barrier is read after it by other threads
-// of the same workgroup (one CU, one L1), which __syncthreads orders.
+// A phase gives each thread at most one cell (BOT_MAX_CELLS = 1024): out of
+// range threads are clamped to the last cell and do not store, so the code
+// has no branch around the loads. (Taking 4096-cell levels here, with four
+// cells per thread, measured slower than their per-level kernels.) Global
+// memory written before a barrier is read after it by other threads of the
+// same workgroup (one CU, one L1), which __syncthreads orders.
 namespace {
 
+constexpr int BT_THREADS = 1024;
+constexpr int BT_CPT = 1;  // cells per thread and phase
+
+// q = p / d for 0 <= p < 2^24 from a float reciprocal, corrected to exact
+__device__ __forceinline__ int bt_div(int p, int d, float inv) {
+    int q = (int)((float)p * inv);
+    if (q * d > p) q--;
+    if ((q + 1) * d <= p) q++;
+    return q;
+}
+
 __device__ __forceinline__ void bt_ijk(const Geo &g, int p, int &i, int &j, int &k) {
-    i = p % g.n0;
-    const int r = p / g.n0;
-    j = r % g.n1;
-    k = r / g.n1;
+    const float i0 = 1.0f / (float)g.n0, i1 = 1.0f / (float)g.n1;
+    const int r = bt_div(p, g.n0, i0);
+    i = p - r * g.n0;
+    k = bt_div(r, g.n1, i1);
+    j = r - k * g.n1;
+}
+
+// neighbour offsets of cell p (0 where there is no neighbour; `has` bits
+// -x +x -y +y -z +z)
+struct BtNb {
+    int xm, xp, ym, yp, zm, zp;
+    int has;
+};
+
+__device__ __forceinline__ BtNb bt_nb(const Geo &g, int p, int &parity) {
+    int i, j, k;
+    bt_ijk(g, p, i, j, k);
+    parity = (i + j + k) & 1;
+    const int n0 = g.n0, plane = g.n0 * g.n1;
+    const int im = nbm(i, g.n0, g.p0), ip = nbp(i, g.n0, g.p0);
+    const int jm = nbm(j, g.n1, g.p1), jp = nbp(j, g.n1, g.p1);
+    const int km = nbm(k, g.n2, g.p2), kp = nbp(k, g.n2, g.p2);
+    BtNb b;
+    b.has = (im >= 0) | ((ip >= 0) << 1) | ((jm >= 0) << 2) | ((jp >= 0) << 3) | ((km >= 0) << 4) | ((kp >= 0) << 5);
+    b.xm = im >= 0 ? p - i + im : p;
+    b.xp = ip >= 0 ? p - i + ip : p;
+    b.ym = jm >= 0 ? p + (jm - j) * n0 : p;
+    b.yp = jp >= 0 ? p + (jp - j) * n0 : p;
+    b.zm = km >= 0 ? p + (km - k) * plane : p;
+    b.zp = kp >= 0 ? p + (kp - k) * plane : p;
+    return b;
 }
 
 // in-place red/black phase: cells of `colour` get (f - sum a v_nb) / c
 // (zero: the iterate is zero, so the cell gets f / c)
 template <typename T>
-__device__ void bt_colour(const BotLevel<T> &L, const T *__restrict__ f, T *v, int colour, bool zero) {
-    const Geo &g = L.g;
-    const int n0 = g.n0, plane = g.n0 * g.n1;
-    for (int p = threadIdx.x; p < (int)g.N; p += blockDim.x) {
-        int i, j, k;
-        bt_ijk(g, p, i, j, k);
-        if (((i + j + k) & 1) != colour) continue;
-        T sum = f[p];
-        if (!zero) {
-            const int im = nbm(i, g.n0, g.p0), ip = nbp(i, g.n0, g.p0);
-            const int jm = nbm(j, g.n1, g.p1), jp = nbp(j, g.n1, g.p1);
-            const int km = nbm(k, g.n2, g.p2), kp = nbp(k, g.n2, g.p2);
-            const int row = p - i, pl = p - j * n0;
-            if (im >= 0) sum -= L.a0[p] * v[row + im];
-            if (ip >= 0) sum -= L.a0[row + ip] * v[row + ip];
-            if (jm >= 0) sum -= L.a1[p] * v[pl + jm * n0];
-            if (jp >= 0) sum -= L.a1[pl + jp * n0] * v[pl + jp * n0];
-            if (km >= 0) sum -= L.a2[p] * v[p + (km - k) * plane];
-            if (kp >= 0) sum -= L.a2[p + (kp - k) * plane] * v[p + (kp - k) * plane];
+__device__ void bt_colour(const BotLevel<T> &L, T *v, int colour, bool zero) {
+    const int N = (int)L.g.N;
+    for (int base = 0; base < N; base += BT_THREADS * BT_CPT) {
+        T res[BT_CPT];
+        int pp[BT_CPT];
+        bool st[BT_CPT];
+#pragma unroll
+        for (int m = 0; m < BT_CPT; m++) {
+            const int q = base + m * BT_THREADS + (int)threadIdx.x;
+            const int p = q < N ? q : N - 1;
+            int par;
+            const BtNb b = bt_nb(L.g, p, par);
+            T sum = L.f[p];
+            if (!zero) {
+                const T vxm = v[b.xm], vxp = v[b.xp], vym = v[b.ym], vyp = v[b.yp], vzm = v[b.zm], vzp = v[b.zp];
+                const T axm = L.a0[p], axp = L.a0[b.xp], aym = L.a1[p], ayp = L.a1[b.yp], azm = L.a2[p],
+                        azp = L.a2[b.zp];
+                if (b.has & 1) sum -= axm * vxm;
+                if (b.has & 2) sum -= axp * vxp;
+                if (b.has & 4) sum -= aym * vym;
+                if (b.has & 8) sum -= ayp * vyp;
+                if (b.has & 16) sum -= azm * vzm;
+                if (b.has & 32) sum -= azp * vzp;
+            }
+            res[m] = sum / L.c[p];
+            pp[m] = p;
+            st[m] = q < N && par == colour;
         }
-        v[p] = sum / L.c[p];
+#pragma unroll
+        for (int m = 0; m < BT_CPT; m++)
+            if (st[m]) v[pp[m]] = res[m];
     }
 }
 
 // w = f - A v
 template <typename T>
-__device__ void bt_resid(const BotLevel<T> &L, const T *__restrict__ f, const T *v, T *w) {
-    const Geo &g = L.g;
-    const int n0 = g.n0, plane = g.n0 * g.n1;
-    for (int p = threadIdx.x; p < (int)g.N; p += blockDim.x) {
-        int i, j, k;
-        bt_ijk(g, p, i, j, k);
-        const int im = nbm(i, g.n0, g.p0), ip = nbp(i, g.n0, g.p0);
-        const int jm = nbm(j, g.n1, g.p1), jp = nbp(j, g.n1, g.p1);
-        const int km = nbm(k, g.n2, g.p2), kp = nbp(k, g.n2, g.p2);
-        const int row = p - i, pl = p - j * n0;
-        T t = L.c[p] * v[p];
-        if (im >= 0) t += L.a0[p] * v[row + im];
-        if (ip >= 0) t += L.a0[row + ip] * v[row + ip];
-        if (jm >= 0) t += L.a1[p] * v[pl + jm * n0];
-        if (jp >= 0) t += L.a1[pl + jp * n0] * v[pl + jp * n0];
-        if (km >= 0) t += L.a2[p] * v[p + (km - k) * plane];
-        if (kp >= 0) t += L.a2[p + (kp - k) * plane] * v[p + (kp - k) * plane];
-        w[p] = f[p] - t;
+__device__ void bt_resid(const BotLevel<T> &L, const T *v, T *w) {
+    const int N = (int)L.g.N;
+    for (int base = 0; base < N; base += BT_THREADS * BT_CPT) {
+#pragma unroll
+        for (int m = 0; m < BT_CPT; m++) {
+            const int q = base + m * BT_THREADS + (int)threadIdx.x;
+            const int p = q < N ? q : N - 1;
+            int par;
+            const BtNb b = bt_nb(L.g, p, par);
+            const T vc = v[p], vxm = v[b.xm], vxp = v[b.xp], vym = v[b.ym], vyp = v[b.yp], vzm = v[b.zm],
+                    vzp = v[b.zp];
+            T t = L.c[p] * vc;
+            if (b.has & 1) t += L.a0[p] * vxm;
+            if (b.has & 2) t += L.a0[b.xp] * vxp;
+            if (b.has & 4) t += L.a1[p] * vym;
+            if (b.has & 8) t += L.a1[b.yp] * vyp;
+            if (b.has & 16) t += L.a2[p] * vzm;
+            if (b.has & 32) t += L.a2[b.zp] * vzp;
+            if (q < N) w[p] = L.f[p] - t;
+        }
     }
 }
 
@@ -668,25 +722,32 @@ __device__ __forceinline__ int bt_per(const Geo &g, int d) { return d == 0 ? g.p
 template <typename T>
 __device__ void bt_gather(const Geo &F, const Geo &C, int d, const T *r, const T *__restrict__ pm,
                           const T *__restrict__ pp, T *fc) {
-    for (int pc = threadIdx.x; pc < (int)C.N; pc += blockDim.x) {
-        int I, J, K;
-        bt_ijk(C, pc, I, J, K);
-        const int i = d == 0 ? 2 * I : I, j = d == 1 ? 2 * J : J, k = d == 2 ? 2 * K : K;
-        const int e = d == 0 ? i : (d == 1 ? j : k);
-        const int nd = bt_dim(F, d), per = bt_per(F, d);
-        const int stride = d == 0 ? 1 : (d == 1 ? F.n0 : F.n0 * F.n1);
-        const int pf = (k * F.n1 + j) * F.n0 + i;
-        const int l = nbm(e, nd, per);
-        T s = r[pf];
-        if (e + 1 < nd) s += pm[pc] * r[pf + stride];
-        if (l >= 0) {
+    const int N = (int)C.N;
+    for (int base = 0; base < N; base += BT_THREADS * BT_CPT) {
+#pragma unroll
+        for (int m = 0; m < BT_CPT; m++) {
+            const int q = base + m * BT_THREADS + (int)threadIdx.x;
+            const int pc = q < N ? q : N - 1;
+            int I, J, K;
+            bt_ijk(C, pc, I, J, K);
+            const int i = d == 0 ? 2 * I : I, j = d == 1 ? 2 * J : J, k = d == 2 ? 2 * K : K;
+            const int e = d == 0 ? i : (d == 1 ? j : k);
+            const int nd = bt_dim(F, d), per = bt_per(F, d);
+            const int stride = d == 0 ? 1 : (d == 1 ? F.n0 : F.n0 * F.n1);
+            const int pf = (k * F.n1 + j) * F.n0 + i;
+            const int l = nbm(e, nd, per);
+            const bool hasr = e + 1 < nd;
             const int Jd = d == 0 ? I : (d == 1 ? J : K);
-            const int jl = nbm(Jd, bt_dim(C, d), bt_per(C, d));
+            const int jl = l >= 0 ? nbm(Jd, bt_dim(C, d), bt_per(C, d)) : Jd;
             const int pl = d == 0 ? (K * C.n1 + J) * C.n0 + jl
                                   : (d == 1 ? (K * C.n1 + jl) * C.n0 + I : (jl * C.n1 + J) * C.n0 + I);
-            s += pp[pl] * r[pf + (l - e) * stride];
+            const T r0 = r[pf], rr = r[hasr ? pf + stride : pf], rl = r[l >= 0 ? pf + (l - e) * stride : pf];
+            const T wm = pm[pc], wp = pp[pl];
+            T s = r0;
+            if (hasr) s += wm * rr;
+            if (l >= 0) s += wp * rl;
+            if (q < N) fc[pc] = s;
         }
-        fc[pc] = s;
     }
 }
 
@@ -694,54 +755,63 @@ __device__ void bt_gather(const Geo &F, const Geo &C, int d, const T *r, const T
 template <typename T>
 __device__ void bt_prolong(const Geo &F, const Geo &C, int d, const T *__restrict__ pm, const T *__restrict__ pp,
                            const T *vc, T *v) {
-    for (int p = threadIdx.x; p < (int)F.N; p += blockDim.x) {
-        int i, j, k;
-        bt_ijk(F, p, i, j, k);
-        const int fd = d == 0 ? i : (d == 1 ? j : k);
-        const int Jd = fd >> 1;
-        const int I = d == 0 ? Jd : i, J = d == 1 ? Jd : j, K = d == 2 ? Jd : k;
-        const int pl = (K * C.n1 + J) * C.n0 + I;
-        T add;
-        if ((fd & 1) == 0) {
-            add = vc[pl];
-        } else {
-            add = pm[pl] * vc[pl];
+    const int N = (int)F.N;
+    for (int base = 0; base < N; base += BT_THREADS * BT_CPT) {
+#pragma unroll
+        for (int m = 0; m < BT_CPT; m++) {
+            const int q = base + m * BT_THREADS + (int)threadIdx.x;
+            const int p = q < N ? q : N - 1;
+            int i, j, k;
+            bt_ijk(F, p, i, j, k);
+            const int fd = d == 0 ? i : (d == 1 ? j : k);
+            const int Jd = fd >> 1;
+            const int I = d == 0 ? Jd : i, J = d == 1 ? Jd : j, K = d == 2 ? Jd : k;
+            const int pl = (K * C.n1 + J) * C.n0 + I;
             const int jr = nbp(Jd, bt_dim(C, d), bt_per(C, d));
-            if (jr >= 0) {
-                const int pr = d == 0 ? (K * C.n1 + J) * C.n0 + jr
-                                      : (d == 1 ? (K * C.n1 + jr) * C.n0 + I : (jr * C.n1 + J) * C.n0 + I);
-                add += pp[pl] * vc[pr];
+            const bool odd = fd & 1, hasr = jr >= 0;
+            const int pr = !hasr ? pl
+                                 : (d == 0 ? (K * C.n1 + J) * C.n0 + jr
+                                           : (d == 1 ? (K * C.n1 + jr) * C.n0 + I : (jr * C.n1 + J) * C.n0 + I));
+            const T v0 = v[p], c0 = vc[pl], c1 = vc[pr], wm = pm[pl], wp = pp[pl];
+            T add;
+            if (!odd) {
+                add = c0;
+            } else {
+                add = wm * c0;
+                if (hasr) add += wp * c1;
             }
+            if (q < N) v[p] = v0 + add;
         }
-        v[p] = v[p] + add;
     }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(1024) k_bottom(BotArgs<T> a) {
+__global__ void __launch_bounds__(BT_THREADS) k_bottom(BotArgs<T> a) {
     const int nl = a.nlev;
     // down: pre-smoothing, residual, restriction
     for (int l = 0; l < nl - 1; l++) {
         const BotLevel<T> &L = a.L[l];
         for (int sw = 0; sw < a.npre; sw++) {
-            bt_colour<T>(L, L.f, L.v, 0, sw == 0);
+            bt_colour<T>(L, L.v, 0, sw == 0);
             __syncthreads();
-            bt_colour<T>(L, L.f, L.v, 1, false);
+            bt_colour<T>(L, L.v, 1, false);
             __syncthreads();
         }
-        bt_resid<T>(L, L.f, L.v, L.w);
+        bt_resid<T>(L, L.v, L.w);
         __syncthreads();
         bt_gather<T>(L.g, a.L[l + 1].g, L.cdir, L.w, a.L[l + 1].pm, a.L[l + 1].pp, a.L[l + 1].f);
         __syncthreads();
     }
-    // coarsest: v = Ainv f (FP64 accumulation, fixed order)
+    // coarsest: v = Ainv f, one wavefront per row (lane-strided columns, as k_dense_t)
     {
         const BotLevel<T> &C = a.L[nl - 1];
         const int n = (int)C.g.N;
-        for (int row = threadIdx.x; row < n; row += blockDim.x) {
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        for (int row = wave; row < n; row += BT_THREADS / 64) {
             double s = 0.0;
-            for (int jj = 0; jj < n; jj++) s += a.Ainv[(long long)row * n + jj] * (double)C.f[jj];
-            C.v[row] = (T)s;
+            for (int jj = lane; jj < n; jj += 64) s += a.Ainv[(long long)row * n + jj] * (double)C.f[jj];
+            s = wave_sum(s);
+            if (lane == 0) C.v[row] = (T)s;
         }
         __syncthreads();
     }
@@ -751,9 +821,9 @@ __global__ void __launch_bounds__(1024) k_bottom(BotArgs<T> a) {
         bt_prolong<T>(L.g, a.L[l + 1].g, L.cdir, a.L[l + 1].pm, a.L[l + 1].pp, a.L[l + 1].v, L.v);
         __syncthreads();
         for (int sw = 0; sw < a.npost; sw++) {
-            bt_colour<T>(L, L.f, L.v, 1, false);
+            bt_colour<T>(L, L.v, 1, false);
             __syncthreads();
-            bt_colour<T>(L, L.f, L.v, 0, false);
+            bt_colour<T>(L, L.v, 0, false);
             __syncthreads();
         }
     }
@@ -763,7 +833,7 @@ __global__ void __launch_bounds__(1024) k_bottom(BotArgs<T> a) {
 
 template <typename T>
 void launch_bottom(hipStream_t s, const BotArgs<T> &a) {
-    hipLaunchKernelGGL(k_bottom<T>, dim3(1), dim3(1024), 0, s, a);
+    hipLaunchKernelGGL(k_bottom<T>, dim3(1), dim3(BT_THREADS), 0, s, a);
 }
 
 // ------------------------------------------------------------------ launchers
