@@ -237,13 +237,11 @@ def sphere_vf(cfg, centres, radii, nsub=4):
     return VF
 
 
-def spheres_two_phase(n=256, seed=3003, vel_seed=1002, nsph=50, rmin=0.05, rmax=0.15, rho_l=1000.0, rho_g=1.0,
-                      nsub=4, no=1):
-    x = uniform(n)
-    cfg = Config(x, x, x, no=no, xper=True, yper=True, zper=True, name="spheres_two_phase")
-    met = incomp.Metrics(cfg)
-    centres, radii = sphere_set(nsph, rmin, rmax, seed)
-    vf = sphere_vf(cfg, centres, radii, nsub=nsub)
+def face_densities(cfg, vf, rho_l, rho_g):
+    """Cell density VF rho_l + (1 - VF) rho_g (periodic ghosts synced), then the
+    arithmetic mean of the two cells on each face: rho_U(i) = (rho(i-1) + rho(i)) / 2
+    (tpns_class.f90:2610-2611 with even sub-cell volumes). Faces without a lower
+    neighbour in the array keep 1."""
     rho = cfg.field()
     rho[cfg.interior] = vf * rho_l + (1.0 - vf) * rho_g
     cfg.sync(rho)
@@ -256,6 +254,17 @@ def spheres_two_phase(n=256, seed=3003, vel_seed=1002, nsph=50, rmin=0.05, rmax=
         hi[ax] = slice(1, None)
         rf[tuple(hi)] = 0.5 * (rho[tuple(lo)] + rho[tuple(hi)])
         rho_faces.append(rf)
+    return rho_faces
+
+
+def spheres_two_phase(n=256, seed=3003, vel_seed=1002, nsph=50, rmin=0.05, rmax=0.15, rho_l=1000.0, rho_g=1.0,
+                      nsub=4, no=1):
+    x = uniform(n)
+    cfg = Config(x, x, x, no=no, xper=True, yper=True, zper=True, name="spheres_two_phase")
+    met = incomp.Metrics(cfg)
+    centres, radii = sphere_set(nsph, rmin, rmax, seed)
+    vf = sphere_vf(cfg, centres, radii, nsub=nsub)
+    rho_faces = face_densities(cfg, vf, rho_l, rho_g)
     opr = incomp.laplacian(met, rho_faces=rho_faces)
     U, V, W = _gaussian_faces(cfg, _rng(vel_seed))
     div = incomp.get_div(met, U, V, W)

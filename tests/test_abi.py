@@ -164,3 +164,26 @@ def test_temporal_predictor_recovers_a_rotation():
     x = learned.extrapolate(sols[-3:], alpha)
     exact = np.cos(w * 30) * v + np.sin(w * 30) * u
     assert float(torch.max(torch.abs(x - exact))) <= 1e-9
+
+
+def test_projection_exports():
+    txt = open(os.path.join(ROOT, "include", "ngapproj.h")).read()
+    names = sorted(set(re.findall(r"^\s*int\s+(ngap_proj_\w+)\s*\(", txt, re.M)))
+    from nga2_amd import proj
+
+    L = proj._bind(ctypes.CDLL(_lib.LIB_PATH))
+    assert set(names) == set(proj.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_face_densities_arithmetic_mean():
+    cfg = Config(uniform(6), uniform(5), uniform(4), no=1, xper=True, yper=True, zper=True)
+    rng = np.random.default_rng(0)
+    vf = rng.uniform(0, 1, (6, 5, 4))
+    ru, rv, rw = inputs.face_densities(cfg, vf, 1000.0, 1.0)
+    rho = vf * 1000.0 + (1 - vf) * 1.0
+    # interior face between cells (i-1, i) along x, with the periodic image at i = 0
+    assert np.isclose(ru[1 + 2, 1 + 3, 1 + 1], 0.5 * (rho[1, 3, 1] + rho[2, 3, 1]))
+    assert np.isclose(ru[1 + 0, 1 + 0, 1 + 0], 0.5 * (rho[5, 0, 0] + rho[0, 0, 0]))
+    assert np.isclose(rw[1 + 1, 1 + 2, 1 + 3], 0.5 * (rho[1, 2, 2] + rho[1, 2, 3]))
