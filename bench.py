@@ -11,11 +11,15 @@ Outside the timed region the line also carries:
   mixed against plain FP64;
 - the learned-guess sequence (128^3 periodic, 200 correlated right-hand
   sides): train on steps 0-149, then iterations and wall time on steps
-  150-199 against the previous-step guess (nga2_amd.learned).
+  150-199 against the previous-step guess (nga2_amd.learned);
+- the drifting-spheres two-phase sequence (256^3, 200 spheres): operator
+  assembled on the device every step, setup + solve of the mixed solve from
+  the learned prediction against plain FP64 (nga2_amd.drift).
 
   python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
                   [--no-cpu] [--cpu-threads T] [--roofline-reps R]
                   [--no-two-phase] [--no-sequence] [--seq-n 128] [--seq-m 8] [--seq-epochs 6]
+                  [--no-drift] [--drift-n 256] [--drift-steps 24]
 """
 from __future__ import annotations
 
@@ -52,6 +56,9 @@ def parse():
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-two-phase", action="store_true", help="skip the 256^3 two-phase spheres leg")
     ap.add_argument("--no-sequence", action="store_true", help="skip the learned-guess sequence leg")
+    ap.add_argument("--no-drift", action="store_true", help="skip the drifting-spheres two-phase sequence leg")
+    ap.add_argument("--drift-n", type=int, default=256)
+    ap.add_argument("--drift-steps", type=int, default=24)
     ap.add_argument("--seq-n", type=int, default=128)
     ap.add_argument("--seq-m", type=int, default=8)
     ap.add_argument("--seq-epochs", type=int, default=6)
@@ -279,6 +286,17 @@ def main():
         sequence["speedup_learned_vs_prev"] = round(ev["ms_prev"] / ev["ms_learned"], 3)
         sequence["speedup_extrap_vs_prev"] = round(ev["ms_prev"] / ev["ms_extrap"], 3)
 
+    drift_res = None
+    if not args.no_drift:
+        from nga2_amd import drift
+
+        log(f"[bench] drifting-spheres sequence ({args.drift_n}^3, 200 spheres, density ratio 1000)")
+        drift_res = drift.run(n=args.drift_n, steps=args.drift_steps, fit_steps=16, m=6, drift_cells=0.5,
+                              oracle_steps=() if args.no_cpu else (args.drift_steps - 1,), log=log)
+        drift_res["workload"] = (f"{args.drift_n}^3 periodic, 200 spheres r 0.02-0.08 drifting 0.5 cell/step (rms), "
+                                 "density ratio 1000, operator assembled on the device every step, fixed RHS; "
+                                 "setup + solve per step, rcvg 1e-10")
+
     steps = args.steps
     el = mixed["elapsed"]
     value = steps / el
@@ -304,6 +322,8 @@ def main():
         out["two_phase_256"] = two_phase
     if sequence:
         out["learned_sequence"] = sequence
+    if drift_res:
+        out["two_phase_drift"] = drift_res
     print(json.dumps(out), flush=True)
     return 0
 
