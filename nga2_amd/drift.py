@@ -151,10 +151,21 @@ class DriftingSpheres:
         return (self.centres + step * self.vel) % 1.0
 
     def operator(self, step):
-        """Assemble the step's operator into self.opr (device); returns VF."""
+        """Assemble the step's operator into self.opr (device); returns VF.
+        self.last_ms gets the times of the VF generation (input synthesis, a
+        stand-in for the VOF transport) and of the face densities + operator
+        assembly (the device projection)."""
+        import torch
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         vf = sphere_vf_device(self.n, self.positions(step), self.radii, device=self.dev)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         rf = face_densities_device(self.cfg, vf, self.rho_l, self.rho_g)
         self.proj.laplacian(self.opr, rf)
+        torch.cuda.synchronize()
+        self.last_ms = {"vf_ms": (t1 - t0) * 1e3, "laplacian_ms": (time.perf_counter() - t1) * 1e3}
         return vf
 
 
@@ -189,6 +200,7 @@ def run(n=256, steps=40, fit_steps=16, m=6, nsph=200, drift_cells=0.5, log=print
     recs = []
     alpha = None
     t_asm = []
+    t_vf, t_lap = [], []
     res_buf = torch.zeros(N, dtype=torch.float64, device=dev)
     b_norm = float(torch.linalg.vector_norm(learned.interior_flat(seq.rhs, cfg)))
 
@@ -201,6 +213,8 @@ def run(n=256, steps=40, fit_steps=16, m=6, nsph=200, drift_cells=0.5, log=print
         seq.operator(step)
         torch.cuda.synchronize()
         t_asm.append((time.perf_counter() - ta) * 1e3)
+        t_vf.append(seq.last_ms["vf_ms"])
+        t_lap.append(seq.last_ms["laplacian_ms"])
         prev = hist[-1] if hist else torch.zeros(N, dtype=torch.float64, device=dev)
         x_prev, s_prev = timed(ds64, FP64, prev)
         x_prev = learned._mean_free(x_prev)
@@ -252,7 +266,8 @@ def run(n=256, steps=40, fit_steps=16, m=6, nsph=200, drift_cells=0.5, log=print
         return float(np.median([r[key][sub] for r in recs]))
 
     out = {"n": n, "spheres": nsph, "steps": steps, "fit_steps": fit_steps, "m": m, "drift_cells": drift_cells,
-           "assembly_ms": float(np.median(t_asm))}
+           "assembly_ms": float(np.median(t_asm)), "vf_generation_ms": float(np.median(t_vf)),
+           "face_densities_laplacian_ms": float(np.median(t_lap))}
     for k in ("zero_fp64", "prev_fp64", "learned_mixed"):
         out[k] = {"it": med(k, "it"), "setup_ms": med(k, "setup_ms"), "solve_ms": med(k, "solve_ms"),
                   "max_true_rerr": max(r[k]["true_rerr"] for r in recs)}
