@@ -14,12 +14,14 @@ Outside the timed region the line also carries:
   150-199 against the previous-step guess (nga2_amd.learned);
 - the drifting-spheres two-phase sequence (256^3, 200 spheres): operator
   assembled on the device every step, setup + solve of the mixed solve from
-  the learned prediction against plain FP64 (nga2_amd.drift).
+  the learned prediction against plain FP64 (nga2_amd.drift);
+- fourier3d, the FFT direct solve, at 256^3 triply periodic (nga2_amd.fourier),
+  against the PCG solve of the same system.
 
   python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
                   [--no-cpu] [--cpu-threads T] [--roofline-reps R]
                   [--no-two-phase] [--no-sequence] [--seq-n 128] [--seq-m 8] [--seq-epochs 6]
-                  [--no-drift] [--drift-n 256] [--drift-steps 24]
+                  [--no-drift] [--drift-n 256] [--drift-steps 24] [--no-fourier]
 """
 from __future__ import annotations
 
@@ -57,6 +59,7 @@ def parse():
     ap.add_argument("--no-two-phase", action="store_true", help="skip the 256^3 two-phase spheres leg")
     ap.add_argument("--no-sequence", action="store_true", help="skip the learned-guess sequence leg")
     ap.add_argument("--no-drift", action="store_true", help="skip the drifting-spheres two-phase sequence leg")
+    ap.add_argument("--no-fourier", action="store_true", help="skip the fourier3d (FFT direct solve) leg")
     ap.add_argument("--drift-n", type=int, default=256)
     ap.add_argument("--drift-steps", type=int, default=24)
     ap.add_argument("--seq-n", type=int, default=128)
@@ -160,6 +163,69 @@ def two_phase_leg(args, dev, solves=3):
     res["speedup_solve_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
     res["speedup_setup_solve_mixed_vs_fp64"] = round(
         (res["fp64"]["solve_ms"] + res["fp64"]["setup_ms"]) / (res["mixed"]["solve_ms"] + res["mixed"]["setup_ms"]), 3)
+    return res
+
+
+def fourier_leg(args, dev, solves=10):
+    """fourier3d on the GPU at n^3 periodic uniform (configuration 1's recipe at
+    the metric's size): setup and solve time, the true residual, and the PCG
+    solve of the same system (mixed) for comparison."""
+    import torch
+
+    from nga2_amd import fourier, inputs
+    from nga2_amd._lib import MIXED
+    from nga2_amd.linsol import DeviceSolve
+
+    t0 = time.time()
+    P = inputs.periodic_gaussian(n=args.n)
+    log(f"[bench] generated {P.name} in {time.time() - t0:.1f}s")
+    cfg = P.cfg
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
+    d_x = torch.zeros_like(d_rhs)
+    ds = fourier.DeviceFourier(cfg)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    ds.setup(d_opr)
+    setup_ms = (time.perf_counter() - t) * 1e3
+    ds.solve(d_rhs, d_x)  # warm-up
+    wall, dev_ms = [], []
+    for _ in range(solves):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ds.solve(d_rhs, d_x)
+        wall.append((time.perf_counter() - t) * 1e3)
+        dev_ms.append(ds.last_ms())
+    ds.destroy()
+    # true residual with the exact operator, and the PCG solution of the same system
+    ps = DeviceSolve(cfg)
+    ps.setup(d_opr, maxit=200, rcvg=args.rcvg, precision=MIXED)
+    d_r = torch.zeros(cfg.nx * cfg.ny * cfg.nz, dtype=torch.float64, device=dev)
+    rnorm = ps.residual(d_rhs, d_x, d_r)
+    from nga2_amd import learned
+
+    bnorm = float(torch.linalg.vector_norm(learned.interior_flat(d_rhs, cfg)))
+    d_y = torch.zeros_like(d_rhs)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    it, _ = ps.solve(d_rhs, d_y)
+    pcg_ms = (time.perf_counter() - t) * 1e3
+    x = learned._mean_free(learned.interior_flat(d_x, cfg))
+    y = learned._mean_free(learned.interior_flat(d_y, cfg))
+    diff = float(torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(x))
+    ps.destroy()
+    N = cfg.nx * cfg.ny * cfg.nz
+    res = {"workload": f"{P.name}: {cfg.nx}^3 triply periodic, uniform, Gaussian faces seed 1001, direct FFT solve "
+                       "(rocFFT D2Z/Z2D FP64 + HIP gather/divide/scatter), inputs resident in HBM",
+           "setup_ms": round(setup_ms, 3), "solve_ms": round(float(np.median(wall)), 3),
+           "solve_ms_device": round(float(np.median(dev_ms)), 3),
+           "solves_per_s": round(1e3 / float(np.median(wall)), 2),
+           "true_rerr": rnorm / bnorm, "pcg_mixed_same_system": {"iters": it, "solve_ms": round(pcg_ms, 3),
+                                                                 "rel_diff_vs_fft": diff},
+           "speedup_vs_pcg_mixed": round(pcg_ms / float(np.median(wall)), 2),
+           "cells": N}
+    log(f"[bench] fourier3d: solve {res['solve_ms']} ms (device {res['solve_ms_device']} ms), setup {setup_ms:.1f} ms, "
+        f"true rerr {res['true_rerr']:.2e}, PCG mixed {pcg_ms:.1f} ms ({it} it), diff {diff:.2e}")
     return res
 
 
@@ -286,6 +352,10 @@ def main():
         sequence["speedup_learned_vs_prev"] = round(ev["ms_prev"] / ev["ms_learned"], 3)
         sequence["speedup_extrap_vs_prev"] = round(ev["ms_prev"] / ev["ms_extrap"], 3)
 
+    fft_res = None
+    if not args.no_fourier:
+        fft_res = fourier_leg(args, dev)
+
     drift_res = None
     if not args.no_drift:
         from nga2_amd import drift
@@ -324,6 +394,8 @@ def main():
         out["learned_sequence"] = sequence
     if drift_res:
         out["two_phase_drift"] = drift_res
+    if fft_res:
+        out["fourier3d"] = fft_res
     print(json.dumps(out), flush=True)
     return 0
 

@@ -62,13 +62,16 @@ extern "C" {
 #define NGAP_ERR_NOT_INIT (-3)    /* setup before init */
 #define NGAP_ERR_NOT_SETUP (-4)   /* solve before setup (hypre_str_class.f90:330) */
 #define NGAP_ERR_METHOD (-5)      /* method not provided (hypre_str_class.f90:311) */
-#define NGAP_ERR_UNSUPPORTED (-6) /* periodic direction with an odd cell count >= 3 */
+#define NGAP_ERR_UNSUPPORTED (-6) /* PCG: periodic direction with an odd cell count >= 3;
+                                     fourier3d: a direction of more than one cell not periodic */
+#define NGAP_ERR_NOT_CIRCULANT (-7) /* fourier3d: operator not uniform in space */
 #define NGAP_ERR_HIP 1            /* a HIP runtime call failed */
 #define NGAP_ERR_NONFINITE 2      /* NaN/Inf in opr, rhs or the initial guess */
 #define NGAP_ERR_BREAKDOWN 3      /* <p,Ap> <= 0 or non-finite in the FP64 iteration */
 #define NGAP_ERR_ALLOC 4          /* device or host allocation failed */
 #define NGAP_ERR_COARSE 5         /* coarsest-level factorisation failed */
 #define NGAP_ERR_STAGE 6          /* the correction stage returned an error */
+#define NGAP_ERR_SINGULAR 7       /* fourier3d: transformed operator near zero or NaN */
 
 /* ngap_stats.status */
 #define NGAP_STATUS_CONVERGED 0

@@ -17,7 +17,8 @@ FP64, MIXED = 0, 1
 
 OK = 0
 ERR_ARG, ERR_STENCIL, ERR_NOT_INIT, ERR_NOT_SETUP, ERR_METHOD, ERR_UNSUPPORTED = -1, -2, -3, -4, -5, -6
-ERR_HIP, ERR_NONFINITE, ERR_BREAKDOWN, ERR_ALLOC, ERR_COARSE, ERR_STAGE = 1, 2, 3, 4, 5, 6
+ERR_NOT_CIRCULANT = -7
+ERR_HIP, ERR_NONFINITE, ERR_BREAKDOWN, ERR_ALLOC, ERR_COARSE, ERR_STAGE, ERR_SINGULAR = 1, 2, 3, 4, 5, 6, 7
 STAGE_GUESS, STAGE_PRECOND = 1, 2
 STATUS_CONVERGED, STATUS_MAXIT = 0, 1
 

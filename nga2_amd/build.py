@@ -63,6 +63,20 @@ def build_net_lib():
     return out
 
 
+FFT_SOURCES = ["ngap_fft.hip"]
+
+
+def build_fft_lib():
+    """libngapfft.so: fourier3d on the GPU (rocFFT transforms through hipFFT)."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libngapfft.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in FFT_SOURCES]
+    _run([hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-I" + CSRC] + srcs + ["-lhipfft", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 def build_oracle():
     _run(["make", "-B", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "liborc.so")
@@ -98,7 +112,7 @@ def build_fortran():
 
 
 def build_all(fortran=True):
-    out = {"solver": build_solver_lib(), "net": build_net_lib(), "oracle": build_oracle()}
+    out = {"solver": build_solver_lib(), "net": build_net_lib(), "fft": build_fft_lib(), "oracle": build_oracle()}
     if fortran:
         out["fortran"] = build_fortran()
     return out

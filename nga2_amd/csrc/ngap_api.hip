@@ -1460,7 +1460,9 @@ const char *ngap_strerror(int code) {
         case NGAP_ERR_NOT_INIT: return "solver has not been initialised";
         case NGAP_ERR_NOT_SETUP: return "solver has not been setup";
         case NGAP_ERR_METHOD: return "unknown or unsupported solution method";
-        case NGAP_ERR_UNSUPPORTED: return "periodic direction with an odd number of cells";
+        case NGAP_ERR_UNSUPPORTED: return "grid not supported (PCG: periodic direction with an odd number of cells; fourier3d: non-periodic direction)";
+        case NGAP_ERR_NOT_CIRCULANT: return "operator must be uniform in space";
+        case NGAP_ERR_SINGULAR: return "transformed operator has elements near zero or NaN";
         case NGAP_ERR_HIP: return "HIP runtime error";
         case NGAP_ERR_NONFINITE: return "non-finite value in opr, rhs or initial guess";
         case NGAP_ERR_BREAKDOWN: return "PCG breakdown";

@@ -187,3 +187,30 @@ def test_face_densities_arithmetic_mean():
     assert np.isclose(ru[1 + 2, 1 + 3, 1 + 1], 0.5 * (rho[1, 3, 1] + rho[2, 3, 1]))
     assert np.isclose(ru[1 + 0, 1 + 0, 1 + 0], 0.5 * (rho[5, 0, 0] + rho[0, 0, 0]))
     assert np.isclose(rw[1 + 1, 1 + 2, 1 + 3], 0.5 * (rho[1, 2, 2] + rho[1, 2, 3]))
+
+
+def test_fourier_exports_and_host_checks():
+    """libngapfft.so loads, exports what ngapfft.h declares, and its create
+    rejects bad grids before touching the device."""
+    from nga2_amd import fourier
+    from nga2_amd.linsol import grid_of
+
+    txt = open(os.path.join(ROOT, "include", "ngapfft.h")).read()
+    names = sorted(set(re.findall(r"^\s*int\s+(ngap_fft_\w+)\s*\(", txt, re.M)))
+    assert set(names) == set(fourier.EXPORTS)
+    L = fourier.lib()
+    for n in names:
+        assert hasattr(L, n), n
+        assert getattr(L, n).argtypes is not None, n
+    h = ctypes.c_void_p()
+    stc = np.asfortranarray(incomp.STC7)
+    sp = stc.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+    cfg = Config(uniform(8), uniform(8), uniform(8), no=1, yper=False)  # walls in y
+    assert L.ngap_fft_create(ctypes.byref(h), ctypes.byref(grid_of(cfg)), 7, sp) == _lib.ERR_UNSUPPORTED
+    g = grid_of(Config(uniform(8), uniform(8), uniform(8), no=1))
+    assert L.ngap_fft_create(ctypes.byref(h), ctypes.byref(g), 0, sp) == _lib.ERR_ARG
+    g.imax_ = 5
+    assert L.ngap_fft_create(ctypes.byref(h), ctypes.byref(g), 7, sp) == _lib.ERR_ARG
+    assert L.ngap_fft_solve(None, None, None) == _lib.ERR_ARG
+    assert "uniform" in _lib.strerror(_lib.ERR_NOT_CIRCULANT)
+    assert "near zero" in _lib.strerror(_lib.ERR_SINGULAR)
