@@ -90,7 +90,7 @@ def flang():
 
 
 FORTRAN_DIR = os.path.join(PKG, "fortran")
-FORTRAN_SOURCES = ["ngap_c.f90", "ngap_str_class.f90"]
+FORTRAN_SOURCES = ["ngap_c.f90", "ngap_str_class.f90", "ngap_fourier3d_class.f90"]
 FORTRAN_TEST_SOURCES = ["standalone/precision.f90", "standalone/config_class.f90", "standalone/linsol_class.f90"]
 
 
@@ -103,12 +103,15 @@ def build_fortran():
         return None
     bdir = os.path.join(LIBDIR, "fortran")
     os.makedirs(bdir, exist_ok=True)
-    srcs = [os.path.join(FORTRAN_DIR, s) for s in FORTRAN_TEST_SOURCES + FORTRAN_SOURCES + ["demo_driver.f90"]]
-    exe = os.path.join(bdir, "ngap_demo")
-    _run([fc, "-O2", "-J", bdir] + srcs + ["-L" + LIBDIR, "-lngapsolve", "-Wl,-rpath," + LIBDIR, "-o", exe + ".tmp"],
-         cwd=bdir)
-    os.replace(exe + ".tmp", exe)
-    return exe
+    srcs = [os.path.join(FORTRAN_DIR, s) for s in FORTRAN_TEST_SOURCES + FORTRAN_SOURCES]
+    libs = ["-L" + LIBDIR, "-lngapsolve", "-lngapfft", "-Wl,-rpath," + LIBDIR]
+    exes = []
+    for drv, name in (("demo_driver.f90", "ngap_demo"), ("fft_demo.f90", "ngap_fft_demo")):
+        exe = os.path.join(bdir, name)
+        _run([fc, "-O2", "-J", bdir] + srcs + [os.path.join(FORTRAN_DIR, drv)] + libs + ["-o", exe + ".tmp"], cwd=bdir)
+        os.replace(exe + ".tmp", exe)
+        exes.append(exe)
+    return exes[0]
 
 
 def build_all(fortran=True):

@@ -44,6 +44,8 @@ module ngap_c
 
    public :: ngap_default_opts,ngap_create,ngap_init,ngap_setup,ngap_solve,ngap_set_tolerance,ngap_set_stage
    public :: ngap_destroy,ngap_last_stats,ngap_strerror_f
+   ! fourier3d on the GPU (libngapfft.so, include/ngapfft.h)
+   public :: ngap_fft_create,ngap_fft_setup,ngap_fft_solve,ngap_fft_destroy
 
    interface
       integer(c_int) function ngap_default_opts(o) bind(C,name='ngap_default_opts')
@@ -103,6 +105,29 @@ module ngap_c
          type(c_ptr), value :: h
          type(ngap_stats_t), intent(out) :: s
       end function ngap_last_stats
+      integer(c_int) function ngap_fft_create(f,g,nst,stc) bind(C,name='ngap_fft_create')
+         import :: c_int,c_ptr,ngap_grid_t
+         type(c_ptr), intent(out) :: f
+         type(ngap_grid_t), intent(in) :: g
+         integer(c_int), value :: nst
+         integer(c_int), intent(in) :: stc(*)
+      end function ngap_fft_create
+      integer(c_int) function ngap_fft_setup(f,opr,circ_tol) bind(C,name='ngap_fft_setup')
+         import :: c_int,c_ptr,c_double
+         type(c_ptr), value :: f
+         real(c_double), intent(in) :: opr(*)
+         real(c_double), value :: circ_tol
+      end function ngap_fft_setup
+      integer(c_int) function ngap_fft_solve(f,rhs,sol) bind(C,name='ngap_fft_solve')
+         import :: c_int,c_ptr,c_double
+         type(c_ptr), value :: f
+         real(c_double), intent(in) :: rhs(*)
+         real(c_double), intent(inout) :: sol(*)
+      end function ngap_fft_solve
+      integer(c_int) function ngap_fft_destroy(f) bind(C,name='ngap_fft_destroy')
+         import :: c_int,c_ptr
+         type(c_ptr), value :: f
+      end function ngap_fft_destroy
       type(c_ptr) function ngap_strerror(code) bind(C,name='ngap_strerror')
          import :: c_int,c_ptr
          integer(c_int), value :: code

@@ -17,6 +17,9 @@ module config_class
       logical :: xper,yper,zper
       logical :: amRoot=.true.
       real(WP) :: dx,dy,dz
+      logical :: uniform_x=.true.,uniform_y=.true.,uniform_z=.true.  !< sgrid_class.f90:266-268 (uniform mesh here)
+      integer :: npx=1,npy=1,npz=1                                    !< one rank
+      real(WP) :: min_meshsize                                        !< config_class.f90:112-125
    contains
       procedure :: sync=>config_sync
    end type config
@@ -37,6 +40,16 @@ contains
       self%imino_=1-no; self%imaxo_=nx+no; self%jmino_=1-no; self%jmaxo_=ny+no; self%kmino_=1-no; self%kmaxo_=nz+no
       self%xper=xper; self%yper=yper; self%zper=zper
       self%dx=Lx/real(nx,WP); self%dy=Ly/real(ny,WP); self%dz=Lz/real(nz,WP)
+      ! geometric mean of the spacings of the directions with more than one cell
+      block
+         integer :: px,py,pz
+         px=merge(0,1,nx.eq.1); py=merge(0,1,ny.eq.1); pz=merge(0,1,nz.eq.1)
+         if (px+py+pz.gt.0) then
+            self%min_meshsize=(self%dx**px*self%dy**py*self%dz**pz)**(1.0_WP/real(px+py+pz,WP))
+         else
+            self%min_meshsize=1.0_WP
+         end if
+      end block
    end function config_from_args
 
    !> Ghost fill: periodic copy in periodic directions, nothing otherwise

@@ -77,12 +77,16 @@ def _uniform(d):
 
 
 def check_grid(cfg):
-    """The constructor's checks (fourier3d_class.f90:146-148): every direction
-    with more than one cell periodic and uniform."""
+    """The constructor's checks (fourier3d_class.f90:144-162): every direction
+    with more than one cell periodic and uniform, and more directions with
+    more than one cell than decomposed ones (none on one GPU), so a 1x1x1
+    grid is refused as the reference refuses it."""
     for n, per, d, ax in ((cfg.nx, cfg.xper, cfg.dx, "x"), (cfg.ny, cfg.yper, cfg.dy, "y"),
                           (cfg.nz, cfg.zper, cfg.dz, "z")):
         if n > 1 and not (per and _uniform(d)):
             raise ValueError(f"[fourier3d constructor] {ax}-direction needs to be periodic and uniform")
+    if sum(n > 1 for n in (cfg.nx, cfg.ny, cfg.nz)) == 0:
+        raise ValueError("[fourier3d constructor] Need at least one NON-decomposed direction")
 
 
 class Fourier3d:

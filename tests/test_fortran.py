@@ -1,7 +1,9 @@
-"""The Fortran drop-in (nga2_amd/fortran/ngap_str_class.f90): it compiles with
-amdflang against the linsol contract, links libngapsolve.so, and its
-self-checking driver solves manufactured problems on the GPU in FP64 and
-mixed precision (periodic box and channel with walls)."""
+"""The Fortran drop-ins (nga2_amd/fortran/ngap_str_class.f90 for hypre_str,
+ngap_fourier3d_class.f90 for fourier3d): they compile with amdflang against the
+linsol contract and link libngapsolve.so / libngapfft.so. The self-checking
+drivers solve manufactured problems on the GPU: PCG in FP64 and mixed
+precision (periodic box and channel with walls), and the FFT direct solve
+(anisotropic 3-D box, one-cell-thick 2-D box)."""
 import os
 import subprocess
 
@@ -10,16 +12,17 @@ import pytest
 from conftest import ROOT
 
 DEMO = os.path.join(ROOT, "nga2_amd", "lib", "fortran", "ngap_demo")
+FFT_DEMO = os.path.join(ROOT, "nga2_amd", "lib", "fortran", "ngap_fft_demo")
 
 
 def _need_demo():
-    if not os.path.exists(DEMO):
+    if not (os.path.exists(DEMO) and os.path.exists(FFT_DEMO)):
         from nga2_amd.build import build_fortran, flang
 
         if flang() is None:
             pytest.skip("no Fortran compiler in this image")
         build_fortran()
-    assert os.path.exists(DEMO)
+    assert os.path.exists(DEMO) and os.path.exists(FFT_DEMO)
 
 
 def test_fortran_shim_builds_and_fails_cleanly_without_gpu():
@@ -33,6 +36,9 @@ def test_fortran_shim_builds_and_fails_cleanly_without_gpu():
     r = subprocess.run([DEMO, "8"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
     assert r.returncode == 2
     assert "[ngap_str init] HIP runtime error" in r.stdout
+    r = subprocess.run([FFT_DEMO, "8"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "[ngap_fourier3d init] HIP runtime error" in r.stdout
 
 
 @pytest.mark.gpu
@@ -44,3 +50,14 @@ def test_fortran_driver_on_gpu():
     assert r.returncode == 0, r.stdout
     assert "NGAP_DEMO OK" in r.stdout
     assert r.stdout.count(" it=") == 4
+
+
+@pytest.mark.gpu
+def test_fortran_fft_driver_on_gpu():
+    _need_demo()
+    print("\n[fortran fft demo 32^3]", flush=True)
+    r = subprocess.run([FFT_DEMO, "32"], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=300)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout
+    assert "NGAP_FFT_DEMO OK" in r.stdout
+    assert r.stdout.count(" it=1 ") == 2

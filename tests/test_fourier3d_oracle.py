@@ -123,6 +123,8 @@ def test_failure_cases():
     with pytest.raises(ValueError):
         fourier.check_grid(Config(x, uniform(8), uniform(8), no=1))  # non-uniform x
     fourier.check_grid(Config(uniform(8), uniform(8), uniform(1), no=1, zper=False))  # one cell in z
+    with pytest.raises(ValueError):
+        fourier.check_grid(Config(uniform(1), uniform(1), uniform(1), no=1))  # no direction to transform
 
 
 def test_min_meshsize_and_tolerance():
