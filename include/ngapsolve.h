@@ -98,6 +98,7 @@ typedef struct ngap_opts {
     int check_finite; /* scan opr/rhs/guess for NaN/Inf, default 1 */
     int verbose;      /* 0 quiet, 1 one line per solve on stderr */
     int graphs;       /* replay the V-cycle as a HIP graph, default 1 */
+    int kdim;         /* GMRES restart length (HYPRE_StructGMRESSetKDim; HYPRE's default 5) */
 } ngap_opts;
 
 typedef struct ngap_stats {
@@ -126,8 +127,12 @@ typedef struct ngap_stats {
 int ngap_default_opts(ngap_opts *o);
 
 /* Replaces hypre_str_from_args (hypre_str_class.f90:68-103). stc is the
- * Fortran array stc(nst,3) in column-major order. Only method NGAP_PCG_PFMG
- * is provided (others -> NGAP_ERR_METHOD at ngap_setup). */
+ * Fortran array stc(nst,3) in column-major order. Methods NGAP_PCG_PFMG and
+ * NGAP_GMRES_PFMG are provided (others -> NGAP_ERR_METHOD at ngap_setup).
+ * NGAP_GMRES_PFMG (:286-303; the implicit velocity solve of
+ * incomp_class.f90:1756-1803) is flexible GMRES(kdim), right-preconditioned
+ * by one V-cycle built on the symmetric part (A + A^T)/2 of the operator,
+ * stopping on the true relative residual ||b - A x||/||b|| <= rcvg. */
 int ngap_create(ngap_t **h, const ngap_grid *g, int nst, const int *stc, int method);
 
 /* Replaces hypre_str_init (:109-145): checks the stencil map and the grid. */

@@ -42,7 +42,7 @@ class Grid(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int), ("coarsest_max", C.c_int),
-                ("check_finite", C.c_int), ("verbose", C.c_int), ("graphs", C.c_int)]
+                ("check_finite", C.c_int), ("verbose", C.c_int), ("graphs", C.c_int), ("kdim", C.c_int)]
 
 
 class Stats(C.Structure):

@@ -3,6 +3,7 @@
 // (src/solver/hypre_str_class.f90:264-284 in the NGA2 tree).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -170,6 +171,115 @@ __global__ void __launch_bounds__(NT) k_exact_resid(Geo g, Ghost gh, Slots sl, c
     }
     double t = block_sum(acc, sm);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// y = A x with the exact 7-array operator (the product of k_exact_resid)
+__global__ void __launch_bounds__(NT) k_exact_apply(Geo g, Ghost gh, Slots sl, const double *__restrict__ opr,
+                                                    const double *__restrict__ x, double *__restrict__ y) {
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
+        int i, j, k;
+        decomp0(g, p, i, j, k);
+        const double *A = opr + gidx(gh, i, j, k) * sl.nst;
+        double s;
+        double a0 = A[sl.s[0]];
+        if (!(fabs(a0) > 0.0)) {
+            s = x[p];
+        } else {
+            s = a0 * x[p];
+            int ip = nbp(i, g.n0, g.p0), im = nbm(i, g.n0, g.p0);
+            int jp = nbp(j, g.n1, g.p1), jm = nbm(j, g.n1, g.p1);
+            int kp = nbp(k, g.n2, g.p2), km = nbm(k, g.n2, g.p2);
+            if (ip >= 0) s += A[sl.s[1]] * x[lin0(g, ip, j, k)];
+            if (im >= 0) s += A[sl.s[2]] * x[lin0(g, im, j, k)];
+            if (jp >= 0) s += A[sl.s[3]] * x[lin0(g, i, jp, k)];
+            if (jm >= 0) s += A[sl.s[4]] * x[lin0(g, i, jm, k)];
+            if (kp >= 0) s += A[sl.s[5]] * x[lin0(g, i, j, kp)];
+            if (km >= 0) s += A[sl.s[6]] * x[lin0(g, i, j, km)];
+        }
+        y[p] = s;
+    }
+}
+
+// GMRES basis kernels. V holds nv vectors of n doubles, ld apart; at most
+// GB_MAX vectors per launch (the caller splits longer bases). Partials of a
+// launch (at most GPART blocks, the cap of grid_for) sit GPART apart.
+constexpr int GB_MAX = 8;
+constexpr int GPART = 1024;
+
+// partial <w, V_i>, i < nv: parts[i * GPART + block]
+__global__ void __launch_bounds__(NT) k_mdot(long long n, const double *__restrict__ w, const double *__restrict__ V,
+                                             long long ld, int nv, double *__restrict__ parts) {
+    __shared__ double sm[NT / 64];
+    double acc[GB_MAX];
+#pragma unroll
+    for (int i = 0; i < GB_MAX; i++) acc[i] = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT) {
+        const double wv = w[p];
+#pragma unroll
+        for (int i = 0; i < GB_MAX; i++)
+            if (i < nv) acc[i] += wv * V[i * ld + p];
+    }
+#pragma unroll
+    for (int i = 0; i < GB_MAX; i++) {
+        if (i >= nv) break;
+        const double t = block_sum(acc[i], sm);
+        if (threadIdx.x == 0) parts[i * GPART + blockIdx.x] = t;
+    }
+}
+
+// out[i] = sum of parts[i * GPART + 0..nb-1] in block order, i < nv
+__global__ void __launch_bounds__(NT) k_fin_multi(const double *__restrict__ parts, int nb, int nv,
+                                                  double *__restrict__ out) {
+    __shared__ double sm[NT / 64];
+    for (int i = 0; i < nv; i++) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += NT) v += parts[i * GPART + b];
+        const double t = block_sum(v, sm);
+        if (threadIdx.x == 0) out[i] = t;
+    }
+}
+
+// w -= sum_i coef[i] V_i (i < nv); with sq, partial ||w||^2 of the result
+__global__ void __launch_bounds__(NT) k_gs_update(long long n, double *__restrict__ w, const double *__restrict__ V,
+                                                  long long ld, int nv, const double *__restrict__ coef,
+                                                  double *__restrict__ sq) {
+    __shared__ double sm[NT / 64];
+    double c[GB_MAX];
+#pragma unroll
+    for (int i = 0; i < GB_MAX; i++) c[i] = i < nv ? coef[i] : 0.0;
+    double acc = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT) {
+        double v = w[p];
+#pragma unroll
+        for (int i = 0; i < GB_MAX; i++)
+            if (i < nv) v -= c[i] * V[i * ld + p];
+        w[p] = v;
+        acc += v * v;
+    }
+    if (sq) {
+        const double t = block_sum(acc, sm);
+        if (threadIdx.x == 0) sq[blockIdx.x] = t;
+    }
+}
+
+// x += sum_i coef[i] Z_i (i < nv)
+__global__ void __launch_bounds__(NT) k_lincomb(long long n, double *__restrict__ x, const double *__restrict__ Z,
+                                                long long ld, int nv, const double *__restrict__ coef) {
+    double c[GB_MAX];
+#pragma unroll
+    for (int i = 0; i < GB_MAX; i++) c[i] = i < nv ? coef[i] : 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT) {
+        double v = x[p];
+#pragma unroll
+        for (int i = 0; i < GB_MAX; i++)
+            if (i < nv) v += c[i] * Z[i * ld + p];
+        x[p] = v;
+    }
+}
+
+__global__ void __launch_bounds__(NT) k_scale_to(long long n, const double *__restrict__ s, double f,
+                                                 double *__restrict__ d) {
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT) d[p] = s[p] * f;
 }
 
 __global__ void __launch_bounds__(NT) k_sumsq(long long n, const double *__restrict__ x, double *__restrict__ part) {
@@ -458,6 +568,12 @@ struct ngap_t {
     void *stage_ctx = nullptr;
     int stage_flags = 0, stage_iters = 0;
     double *st_t = nullptr, *st_e = nullptr;  // stage input and output, allocated on first use
+    // flexible GMRES (method NGAP_GMRES_PFMG): Krylov basis V (kdim+1 vectors),
+    // preconditioned directions Z (kdim), and the per-step scalars
+    int kdim = 0;
+    double *gV = nullptr, *gZ = nullptr;
+    double *gm = nullptr;    // device scalars: [0,64) dots, [64,128) second-pass dots, 128 ||w||^2, [192,256) y
+    double *h_gm = nullptr;  // pinned copy
 };
 
 namespace {
@@ -550,6 +666,11 @@ void free_all(ngap_t *h) {
     dfree(h, h->st_t, sizeof(double) * N);
     dfree(h, h->st_e, sizeof(double) * N);
     h->st_t = h->st_e = nullptr;
+    dfree(h, h->gV, sizeof(double) * N * (h->kdim + 1));
+    dfree(h, h->gZ, sizeof(double) * N * h->kdim);
+    dfree(h, h->gm, sizeof(double) * 256);
+    h->gV = h->gZ = h->gm = nullptr;
+    h->kdim = 0;
     dfree(h, h->parts, sizeof(double) * 4 * MAX_PARTS);
     dfree(h, h->scal, sizeof(double) * NSCAL_ALLOC);
     dfree(h, h->flags, sizeof(int) * 4);
@@ -844,7 +965,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     if (!h) return NGAP_ERR_ARG;
     if (!h->init_done) return NGAP_ERR_NOT_INIT;
     if (!opr) return NGAP_ERR_ARG;
-    if (h->method != NGAP_PCG_PFMG) return NGAP_ERR_METHOD;
+    if (h->method != NGAP_PCG_PFMG && h->method != NGAP_GMRES_PFMG) return NGAP_ERR_METHOD;
     if (maxit < 0 || !(rcvg >= 0.0)) return NGAP_ERR_ARG;
     double t0 = now_ms();
     h->setup_done = false;
@@ -855,6 +976,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     if (op.nu_post < 1) op.nu_post = 1;
     if (op.coarsest_max < 1) op.coarsest_max = 128;
     if (op.precision != NGAP_FP64 && op.precision != NGAP_MIXED) return NGAP_ERR_ARG;
+    if (h->method == NGAP_GMRES_PFMG && (op.kdim < 1 || op.kdim > 30)) return NGAP_ERR_ARG;
     h->opts = op;
     h->maxit = maxit;
     h->rcvg = rcvg;
@@ -868,6 +990,18 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     }
     int rc = ensure_base(h);
     if (rc) return rc;
+    if (h->method == NGAP_GMRES_PFMG && h->kdim != op.kdim) {
+        const size_t N = (size_t)h->g0.N;
+        dfree(h, h->gV, sizeof(double) * N * (h->kdim + 1));
+        dfree(h, h->gZ, sizeof(double) * N * h->kdim);
+        h->gV = h->gZ = nullptr;
+        h->kdim = op.kdim;
+        if (dmalloc(h, (void **)&h->gV, sizeof(double) * N * (h->kdim + 1)) ||
+            dmalloc(h, (void **)&h->gZ, sizeof(double) * N * h->kdim))
+            return NGAP_ERR_ALLOC;
+        if (!h->gm && dmalloc(h, (void **)&h->gm, sizeof(double) * 256)) return NGAP_ERR_ALLOC;
+        if (!h->h_gm) HIPCHK(hipHostMalloc((void **)&h->h_gm, sizeof(double) * 256, hipHostMallocDefault));
+    }
     hipStream_t s = h->stream;
     size_t obytes = sizeof(double) * (size_t)h->nghost * h->nst;
     if (on_device) {
@@ -1229,6 +1363,175 @@ int run_pcg(ngap_t *h) {
     return 0;
 }
 
+// ---------------------------------------------------------------- GMRES
+
+// out[0..nv) = <w, V_i> over the basis in chunks of GB_MAX (device), then
+// w -= sum out[i] V_i; with sq_slot >= 0 the squared norm of the updated w
+// goes to h->gm[sq_slot]
+void cgs_pass(ngap_t *h, double *w, const double *V, int nv, double *out, int sq_slot) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    for (int i0 = 0; i0 < nv; i0 += GB_MAX) {
+        const int m = std::min(GB_MAX, nv - i0);
+        hipLaunchKernelGGL(k_mdot, dim3(nb), dim3(NT), 0, s, N, (const double *)w, V + (size_t)i0 * N, N, m,
+                           h->parts);
+        hipLaunchKernelGGL(k_fin_multi, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, m, out + i0);
+    }
+    // classical Gram-Schmidt: every coefficient from the same w, then one update
+    for (int i0 = 0; i0 < nv; i0 += GB_MAX) {
+        const int m = std::min(GB_MAX, nv - i0);
+        const bool last = i0 + m >= nv;
+        hipLaunchKernelGGL(k_gs_update, dim3(nb), dim3(NT), 0, s, N, w, V + (size_t)i0 * N, N, m,
+                           (const double *)(out + i0), (last && sq_slot >= 0) ? h->parts + 3 * MAX_PARTS : nullptr);
+    }
+    if (sq_slot >= 0)
+        hipLaunchKernelGGL(k_fin_multi, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + 3 * MAX_PARTS), nb, 1,
+                           h->gm + sq_slot);
+}
+
+// z_j = M^-1 v_j into Z_j: one V-cycle (FP64, or FP32 in mixed precision)
+void gmres_precond(ngap_t *h, const double *v, double *zj, bool mixed) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    if (mixed) {
+        launch_to_f32(s, N, v, h->r32);
+        precond(h, true);
+        launch_to_f64(s, N, h->z32, zj);
+    } else {
+        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(NT), 0, s, N, v, h->r);
+        precond(h, false);
+        hipLaunchKernelGGL(k_copy, dim3(nb), dim3(NT), 0, s, N, (const double *)h->z, zj);
+    }
+}
+
+int read_gm(ngap_t *h, int n) {
+    HIPCHK(hipMemcpyAsync(h->h_gm, h->gm, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// Flexible GMRES(kdim), right-preconditioned (HYPRE's StructGMRES with the
+// PFMG preconditioner, hypre_str_class.f90:286-303). Each cycle: v0 = r/|r|;
+// per step z_j = M^-1 v_j (one V-cycle of the symmetric part), w = A z_j
+// with the exact operator, two passes of classical Gram-Schmidt, Givens
+// rotations on the host; then x += Z y. The iteration stops when the
+// rotated residual estimate reaches rcvg ||b||, and the true residual
+// b - A x, recomputed at each cycle start, confirms it.
+int run_gmres(ngap_t *h) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    const bool mixed = h->opts.precision == NGAP_MIXED;
+    const int m = h->kdim;
+    int rc;
+    h->st.fallback = 0;
+    h->st.restarts = 0;
+    h->st.stage_calls = 0;
+    h->st.cnorm_ratio = 0.0;
+    h->st.precision = mixed ? NGAP_MIXED : NGAP_FP64;
+    hipLaunchKernelGGL(k_sumsq, dim3(nb), dim3(NT), 0, s, N, (const double *)h->b, h->parts + MAX_PARTS);
+    hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
+                       (int)S_BB);
+    HIPCHK(hipGetLastError());
+    if ((rc = read_scal(h))) return rc;
+    const double bn = std::sqrt(h->h_scal[S_BB]);
+    if (bn == 0.0) {
+        hipLaunchKernelGGL(k_zero64, dim3(nb), dim3(NT), 0, s, N, h->x);
+        HIPCHK(hipGetLastError());
+        h->st.it = 0;
+        h->st.rerr = h->st.true_rerr = h->st.aerr = 0.0;
+        h->st.status = NGAP_STATUS_CONVERGED;
+        return 0;
+    }
+    const double tol = h->rcvg;
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    auto Hij = [&](int i, int j) -> double & { return H[(size_t)j * (m + 1) + i]; };
+    int it = 0;
+    double rel = 0.0, tr = 0.0;
+    bool converged = false;
+    int cycles = 0;
+    while (true) {
+        // true residual: r = b - A x into V_0
+        exact_residual(h, h->gV, S_RRT);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scal(h))) return rc;
+        const double beta = std::sqrt(h->h_scal[S_RRT]);
+        tr = beta / bn;
+        if (!std::isfinite(tr)) return NGAP_ERR_BREAKDOWN;
+        rel = tr;
+        if (tr <= tol) {
+            converged = true;
+            break;
+        }
+        if (it >= h->maxit) break;
+        if (cycles > 0) h->st.restarts++;
+        cycles++;
+        hipLaunchKernelGGL(k_scale_to, dim3(nb), dim3(NT), 0, s, N, (const double *)h->gV, 1.0 / beta, h->gV);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = beta;
+        int k = 0;  // columns of this cycle
+        for (int j = 0; j < m && it < h->maxit; j++) {
+            double *vj = h->gV + (size_t)j * N, *zj = h->gZ + (size_t)j * N;
+            double *w = h->gV + (size_t)(j + 1) * N;
+            gmres_precond(h, vj, zj, mixed);
+            hipLaunchKernelGGL(k_exact_apply, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots,
+                               (const double *)h->opr, (const double *)zj, w);
+            cgs_pass(h, w, h->gV, j + 1, h->gm, -1);
+            cgs_pass(h, w, h->gV, j + 1, h->gm + 64, 128);
+            HIPCHK(hipGetLastError());
+            if ((rc = read_gm(h, 129))) return rc;
+            for (int i = 0; i <= j; i++) Hij(i, j) = h->h_gm[i] + h->h_gm[64 + i];
+            const double hn = std::sqrt(std::max(h->h_gm[128], 0.0));
+            Hij(j + 1, j) = hn;
+            if (!std::isfinite(hn)) return NGAP_ERR_BREAKDOWN;
+            // apply the previous rotations, then a new one zeroing H(j+1, j)
+            for (int i = 0; i < j; i++) {
+                const double a = Hij(i, j), b = Hij(i + 1, j);
+                Hij(i, j) = cs[i] * a + sn[i] * b;
+                Hij(i + 1, j) = -sn[i] * a + cs[i] * b;
+            }
+            const double a = Hij(j, j), b = Hij(j + 1, j);
+            const double rr = std::hypot(a, b);
+            if (!(rr > 0.0)) return NGAP_ERR_BREAKDOWN;
+            cs[j] = a / rr;
+            sn[j] = b / rr;
+            Hij(j, j) = rr;
+            Hij(j + 1, j) = 0.0;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            it++;
+            k = j + 1;
+            rel = std::fabs(g[j + 1]) / bn;
+            // converged estimate, or the space is invariant (happy breakdown)
+            if (rel <= tol || hn <= 1e-300) break;
+            hipLaunchKernelGGL(k_scale_to, dim3(nb), dim3(NT), 0, s, N, (const double *)w, 1.0 / hn, w);
+        }
+        // y = R^-1 g (upper triangular k x k), x += Z y
+        for (int i = k - 1; i >= 0; i--) {
+            double t = g[i];
+            for (int q = i + 1; q < k; q++) t -= Hij(i, q) * y[q];
+            y[i] = t / Hij(i, i);
+        }
+        for (int i = 0; i < k; i++)
+            if (!std::isfinite(y[i])) return NGAP_ERR_BREAKDOWN;
+        HIPCHK(hipMemcpyAsync(h->gm + 192, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
+        for (int i0 = 0; i0 < k; i0 += GB_MAX) {
+            const int mm = std::min(GB_MAX, k - i0);
+            hipLaunchKernelGGL(k_lincomb, dim3(nb), dim3(NT), 0, s, N, h->x, (const double *)(h->gZ + (size_t)i0 * N),
+                               N, mm, (const double *)(h->gm + 192 + i0));
+        }
+        HIPCHK(hipGetLastError());
+    }
+    h->st.it = it;
+    h->st.rerr = rel;
+    h->st.true_rerr = tr;
+    h->st.aerr = tr * bn;
+    h->st.status = converged ? NGAP_STATUS_CONVERGED : NGAP_STATUS_MAXIT;
+    return 0;
+}
+
 int do_solve(ngap_t *h, const double *rhs, double *sol, bool on_device, int *it, double *rerr, double *aerr) {
     if (!h || !rhs || !sol) return NGAP_ERR_ARG;
     if (!h->setup_done) return NGAP_ERR_NOT_SETUP;
@@ -1268,7 +1571,7 @@ int do_solve(ngap_t *h, const double *rhs, double *sol, bool on_device, int *it,
         if (h->h_scal[S_BAD] > 0.0 || h->h_scal[S_RRT] > 0.0) return NGAP_ERR_NONFINITE;
     }
     HIPCHK(hipGetLastError());
-    int rc = run_pcg(h);
+    int rc = h->method == NGAP_GMRES_PFMG ? run_gmres(h) : run_pcg(h);
     if (rc) return rc;
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, (const double *)h->x, dsol);
     HIPCHK(hipGetLastError());
@@ -1449,6 +1752,7 @@ int ngap_default_opts(ngap_opts *o) {
     o->check_finite = 1;
     o->verbose = 0;
     o->graphs = 1;
+    o->kdim = 5;
     return 0;
 }
 
@@ -1612,6 +1916,7 @@ int ngap_destroy(ngap_t *h) {
         free_all(h);
         if (h->h_scal) (void)hipHostFree(h->h_scal);
         if (h->h_flags) (void)hipHostFree(h->h_flags);
+        if (h->h_gm) (void)hipHostFree(h->h_gm);
         if (h->stream) (void)hipStreamDestroy(h->stream);
     }
     delete h;

@@ -6,10 +6,12 @@
 // solve.
 //
 // The transforms are real-to-complex over the directions with more than one
-// cell, slowest first (z, y, x). They read the interior of the caller's
-// ghosted rhs in place and write the interior of sol in place (strided plan
-// layouts), so a solve is: forward transform, one pass over the half
-// spectrum (zero mode, 1/F, normalisation), inverse transform, ghost fill.
+// cell, slowest first (z, y, x). The forward one reads the interior of the
+// caller's ghosted rhs in place (strided plan layout); the inverse one writes
+// a dense array (rocFFT's two-dimensional complex-to-real plans with a
+// strided output fail to execute), which one pass copies into the interior
+// of sol. A solve is: forward transform, one pass over the half spectrum
+// (zero mode, 1/F, normalisation), inverse transform, copy, ghost fill.
 // The half spectrum (last transformed direction n/2+1 long) holds every mode
 // of the reference's full complex transform, since F(-k) = conj F(k) for a
 // real stencil and a real right-hand side.
@@ -18,6 +20,7 @@
 
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
 #include <vector>
 
 #include "ngapfft.h"
@@ -134,6 +137,15 @@ __global__ void __launch_bounds__(PT) k_divide(long long M, double2 *__restrict_
     }
 }
 
+// interior of a ghosted array from a dense x-fastest one
+__global__ void __launch_bounds__(PT) k_put(Grid g, const double *__restrict__ d, double *__restrict__ a) {
+    for (long long q = blockIdx.x * (long long)PT + threadIdx.x; q < g.N; q += (long long)gridDim.x * PT) {
+        const int i = (int)(q % g.nx);
+        const long long r = q / g.nx;
+        a[ghosted(g, i, (int)(r % g.ny), (int)(r / g.ny))] = d[q];
+    }
+}
+
 // ghost fill of one direction, as cfg%sync does on one rank
 // (pgrid_class.f90:459-516): the 2*w ghost layers of direction ax take the
 // periodic image, over the whole ghosted extent of the other two directions
@@ -190,7 +202,7 @@ struct ngap_fft {
     hipfftHandle fwd = 0, inv = 0;
     bool have_fwd = false, have_inv = false, setup_done = false;
     int *d_stc = nullptr, *d_part = nullptr;
-    double *d_kern = nullptr;  // ghosted-layout scratch for the stencil's transform (setup)
+    double *d_kern = nullptr;  // scratch: the stencil in ghosted layout (setup), the dense solution (solve)
     long long off = 0;         // offset of the first interior cell in a ghosted array
     double2 *d_spec = nullptr, *d_finv = nullptr;
     double *d_io = nullptr;  // ghosted staging for the host entry points
@@ -218,9 +230,17 @@ int dalloc(ngap_fft *F, T **p, size_t n) {
     return 0;
 }
 
+// a failed runtime or transform call: name it on stderr, return NGAP_ERR_HIP
+int fail(const char *where, int code) {
+    std::fprintf(stderr, "[ngap_fft] %s failed (%d)\n", where, code);
+    return NGAP_ERR_HIP;
+}
+
 int fin(ngap_fft *F) {
-    if (hipGetLastError() != hipSuccess) return NGAP_ERR_HIP;
-    return hipStreamSynchronize(F->s) == hipSuccess ? 0 : NGAP_ERR_HIP;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(hipGetErrorString(e), (int)e);
+    e = hipStreamSynchronize(F->s);
+    return e == hipSuccess ? 0 : fail(hipGetErrorString(e), (int)e);
 }
 
 int sum_parts(ngap_fft *F, const int *d, long long *out) {
@@ -244,8 +264,8 @@ int setup_from(ngap_fft *F, const double *d_opr, double tol) {
     hipLaunchKernelGGL(k_zero, dim3(blocks_for(g.No)), dim3(PT), 0, F->s, F->d_kern, g.No);
     hipLaunchKernelGGL(k_place, dim3(1), dim3(64), 0, F->s, g, F->nst, (const int *)F->d_stc, d_opr, F->d_kern);
     if (F->rank > 0) {
-        if (hipfftExecD2Z(F->fwd, F->d_kern + F->off, (hipfftDoubleComplex *)F->d_finv) != HIPFFT_SUCCESS)
-            return NGAP_ERR_HIP;
+        const hipfftResult r = hipfftExecD2Z(F->fwd, F->d_kern + F->off, (hipfftDoubleComplex *)F->d_finv);
+        if (r != HIPFFT_SUCCESS) return fail("hipfftExecD2Z (setup)", (int)r);
     } else if (hipMemcpyAsync(F->d_finv, F->d_kern + F->off, sizeof(double), hipMemcpyDeviceToDevice, F->s) !=
                hipSuccess) {
         return NGAP_ERR_HIP;  // one cell: the transform is the value itself (imaginary part set below)
@@ -264,25 +284,26 @@ int setup_from(ngap_fft *F, const double *d_opr, double tol) {
 
 int solve_on(ngap_fft *F, const double *d_rhs, double *d_sol) {
     const Grid &g = F->g;
-    if (hipEventRecord(F->e0, F->s) != hipSuccess) return NGAP_ERR_HIP;
+    if (hipEventRecord(F->e0, F->s) != hipSuccess) return fail("hipEventRecord", 0);
     if (F->rank > 0) {
         // the plans read the interior of rhs and write the interior of sol in place
-        if (hipfftExecD2Z(F->fwd, const_cast<double *>(d_rhs) + F->off, (hipfftDoubleComplex *)F->d_spec) !=
-            HIPFFT_SUCCESS)
-            return NGAP_ERR_HIP;
+        hipfftResult r = hipfftExecD2Z(F->fwd, const_cast<double *>(d_rhs) + F->off, (hipfftDoubleComplex *)F->d_spec);
+        if (r != HIPFFT_SUCCESS) return fail("hipfftExecD2Z", (int)r);
         hipLaunchKernelGGL(k_divide, dim3(blocks_for(F->M)), dim3(PT), 0, F->s, F->M, F->d_spec,
                            (const double2 *)F->d_finv, 1.0 / ((double)g.nx * g.ny * g.nz));
-        if (hipfftExecZ2D(F->inv, (hipfftDoubleComplex *)F->d_spec, d_sol + F->off) != HIPFFT_SUCCESS)
-            return NGAP_ERR_HIP;
+        r = hipfftExecZ2D(F->inv, (hipfftDoubleComplex *)F->d_spec, F->d_kern);
+        if (r != HIPFFT_SUCCESS) return fail("hipfftExecZ2D", (int)r);
+        hipLaunchKernelGGL(k_put, dim3(blocks_for(g.N)), dim3(PT), 0, F->s, g, (const double *)F->d_kern, d_sol);
     } else {
         hipLaunchKernelGGL(k_zero, dim3(1), dim3(PT), 0, F->s, d_sol + F->off, 1LL);  // the only mode is the zero mode
     }
     sync_all(F->s, g, d_sol);
-    if (hipEventRecord(F->e1, F->s) != hipSuccess) return NGAP_ERR_HIP;
+    if (hipEventRecord(F->e1, F->s) != hipSuccess) return fail("hipEventRecord", 0);
     int rc = fin(F);
     if (rc) return rc;
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, F->e0, F->e1) != hipSuccess) return NGAP_ERR_HIP;
+    const hipError_t e = hipEventElapsedTime(&ms, F->e0, F->e1);
+    if (e != hipSuccess) return fail("hipEventElapsedTime", (int)e);
     F->last_ms = ms;
     return 0;
 }
@@ -334,10 +355,11 @@ int ngap_fft_create(ngap_fft **fp, const ngap_grid *gr, int nst, const int *stc)
     // real side: the ghosted array (embed[d] = stride ratio of consecutive
     // directions, the fastest stride as the element stride); complex side: the
     // dense half spectrum
-    int rembed[3], cembed[3], rstride = r > 0 ? (int)str[r - 1] : 1;
+    int rembed[3], cembed[3], dembed[3], rstride = r > 0 ? (int)str[r - 1] : 1;
     for (int d = 0; d < r; d++) {
         rembed[d] = d == 0 ? dims[0] : (int)(str[d - 1] / str[d]);
         cembed[d] = d == r - 1 ? dims[d] / 2 + 1 : dims[d];
+        dembed[d] = dims[d];
     }
     int rc = hipGetDevice(&F->device) == hipSuccess ? 0 : NGAP_ERR_HIP;
     if (!rc) rc = hipStreamCreateWithFlags(&F->s, hipStreamNonBlocking) == hipSuccess ? 0 : NGAP_ERR_HIP;
@@ -354,8 +376,8 @@ int ngap_fft_create(ngap_fft **fp, const ngap_grid *gr, int nst, const int *stc)
             rc = NGAP_ERR_HIP;
         else
             F->have_fwd = true;
-        if (!rc && hipfftPlanMany(&F->inv, r, dims, cembed, 1, (int)F->M, rembed, rstride, (int)g.No, HIPFFT_Z2D,
-                                  1) != HIPFFT_SUCCESS)
+        if (!rc && hipfftPlanMany(&F->inv, r, dims, cembed, 1, (int)F->M, dembed, 1, (int)g.N, HIPFFT_Z2D, 1) !=
+                       HIPFFT_SUCCESS)
             rc = NGAP_ERR_HIP;
         else if (!rc)
             F->have_inv = true;
@@ -408,9 +430,10 @@ int ngap_fft_solve(ngap_fft *f, const double *rhs, double *sol) {
     // sol goes up too: ghosts of a non-periodic (one-cell) direction keep the caller's values
     if (hipMemcpy(dr, rhs, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(ds, sol, sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess)
-        return NGAP_ERR_HIP;
+        return fail("hipMemcpy (upload)", 0);
     int rc = solve_on(f, dr, ds);
-    if (!rc && hipMemcpy(sol, ds, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess) rc = NGAP_ERR_HIP;
+    if (!rc && hipMemcpy(sol, ds, sizeof(double) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail("hipMemcpy (download)", 0);
     return rc;
 }
 

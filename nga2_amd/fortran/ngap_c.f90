@@ -33,7 +33,7 @@ module ngap_c
    end type ngap_grid_t
 
    type, bind(C), public :: ngap_opts_t
-      integer(c_int) :: precision,nu_pre,nu_post,coarsest_max,check_finite,verbose,graphs
+      integer(c_int) :: precision,nu_pre,nu_post,coarsest_max,check_finite,verbose,graphs,kdim
    end type ngap_opts_t
 
    type, bind(C), public :: ngap_stats_t

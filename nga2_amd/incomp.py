@@ -206,3 +206,45 @@ def shift_p(cfg: Config, P):
     P[it] = sub
     cfg.sync(P)
     return P
+
+
+def convection_diffusion(met: Metrics, faces, rho, visc, dt):
+    """A nonsymmetric 7-point operator of the form NGA2's implicit velocity
+    solve hands to its ``implicit`` linsol (incomp_class.f90:1756-1803:
+    opr(1) = rho plus half a time step of the convective and viscous
+    couplings, per unit volume), written on the pressure cells of ``met``:
+
+        A x = rho x + dt/2 * [ div(u_f * avg(x)) - visc * div(grad x) ]
+
+    with central face averages, the face velocities ``faces`` = (U, V, W)
+    (zeroed on wall faces) and the masked metrics of ``met``. It is a test
+    operator with the structure and scaling of the implicit one (diagonal
+    rho + O(dt), couplings -dt visc/h^2 +- dt u/(4h)), not a restatement of
+    the staggered U/V/W momentum operators. Returns opr(7, ...) over the
+    interior cells, ghost entries 0, stencil order as ``laplacian``.
+    """
+    cfg = met.cfg
+    opr = np.zeros((7,) + cfg.shape, dtype=np.float64, order="F")
+    it = cfg.interior
+    fmasks = (met.umask, met.vmask, met.wmask)
+    diag = np.full(cfg.shape, float(rho))
+    offd = []
+    h = 0.5 * dt
+    for ax in range(3):
+        p0, p1 = met.divp[ax]
+        u0, u1 = met.divu[ax]
+        uf = np.where(fmasks[ax] == 0, np.asarray(faces[ax], dtype=np.float64), 0.0)
+        uf_up = _shift(uf, ax, 1)
+        u0_up = _shift(u0, ax, 1)
+        u1_up = _shift(u1, ax, 1)
+        # convection: p1 * u_up * (x + x_up)/2 + p0 * u_lo * (x_lo + x)/2
+        diag = diag + h * (0.5 * p1 * uf_up + 0.5 * p0 * uf)
+        conv_up = h * 0.5 * p1 * uf_up
+        conv_lo = h * 0.5 * p0 * uf
+        # diffusion: -visc * (p1 * grad_up + p0 * grad_lo)
+        diag = diag - h * visc * (p1 * u0_up + p0 * u1)
+        offd += [conv_up - h * visc * p1 * u1_up, conv_lo - h * visc * p0 * u0]
+    opr[0][it] = diag[it]
+    for s in range(6):
+        opr[1 + s][it] = offd[s][it]
+    return opr

@@ -332,3 +332,23 @@ WORKLOADS = {
     "channel_spectral": channel_spectral,
     "spheres_two_phase": spheres_two_phase,
 }
+
+
+def implicit_problem(cfg, seed=4004, rho=1.0, visc=0.02, dt=None, cfl=1.0):
+    """A nonsymmetric implicit-solve test system on cfg (incomp.convection_diffusion):
+    Gaussian face velocities scaled to |u|max = 1, dt from the convective
+    CFL number (dt = cfl * h_min / |u|max), and a Gaussian right-hand side.
+    Seeded with numpy PCG64(seed)."""
+    rng = _rng(seed)
+    met = incomp.Metrics(cfg)
+    faces = [rng.standard_normal(cfg.shape) for _ in range(3)]
+    umax = max(float(np.abs(f[cfg.interior]).max()) for f in faces)
+    faces = [f / umax for f in faces]
+    h = min(float(np.min(cfg.dx[cfg.interior[0]])), float(np.min(cfg.dy[cfg.interior[1]])),
+            float(np.min(cfg.dz[cfg.interior[2]])))
+    if dt is None:
+        dt = cfl * h
+    opr = incomp.convection_diffusion(met, faces, rho, visc, dt)
+    rhs = cfg.field()
+    rhs[cfg.interior] = rng.standard_normal((cfg.nx, cfg.ny, cfg.nz))
+    return Problem(cfg, opr, rhs, f"implicit_{cfg.nx}x{cfg.ny}x{cfg.nz}", dict(seed=seed, rho=rho, visc=visc, dt=dt))

@@ -33,6 +33,7 @@ CASES = {
     "periodic16": lambda: (lambda P: (P.cfg, P.opr, P.rhs))(inputs.periodic_gaussian(n=16)),
     "aniso12x10x6": lambda: _box((12, 10, 6), (1.3, 0.7, 2.0)),
     "plane16x8x1": lambda: _box((16, 8, 1), (1.0, 0.5, 0.1), (True, True, False)),
+    "plane32x32x1": lambda: _box((32, 32, 1), (1.0, 1.0, 0.1), (True, True, False)),
     "line32x1x1": lambda: _box((32, 1, 1), (2.0, 1.0, 1.0), (True, False, False)),
     "odd15x9x5": lambda: _box((15, 9, 5), (1.0, 1.0, 1.0)),
     "ghosts2_12": lambda: (lambda P: (P.cfg, P.opr, P.rhs))(inputs.periodic_gaussian(n=12, no=2)),

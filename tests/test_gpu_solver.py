@@ -46,7 +46,7 @@ def test_lifecycle_and_misuse():
 def test_unsupported_method_and_stencil():
     print("\n[method/stencil]", flush=True)
     P = inputs.periodic_gaussian(n=8)
-    ps = NgapStr(P.cfg, "Pressure", 3, 7)  # gmres_pfmg is not provided
+    ps = NgapStr(P.cfg, "Pressure", 4, 7)  # smg is not provided
     ps.stc[:] = STC7
     ps.opr[:] = P.opr
     ps.init()
