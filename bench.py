@@ -16,12 +16,14 @@ Outside the timed region the line also carries:
   assembled on the device every step, setup + solve of the mixed solve from
   the learned prediction against plain FP64 (nga2_amd.drift);
 - fourier3d, the FFT direct solve, at 256^3 triply periodic (nga2_amd.fourier),
-  against the PCG solve of the same system.
+  against the PCG solve of the same system;
+- gmres_pfmg on a nonsymmetric implicit-solve system on the 256^3 channel
+  grid, FP64 and with the FP32 V-cycle.
 
   python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
                   [--no-cpu] [--cpu-threads T] [--roofline-reps R]
                   [--no-two-phase] [--no-sequence] [--seq-n 128] [--seq-m 8] [--seq-epochs 6]
-                  [--no-drift] [--drift-n 256] [--drift-steps 24] [--no-fourier]
+                  [--no-drift] [--drift-n 256] [--drift-steps 24] [--no-fourier] [--no-implicit]
 """
 from __future__ import annotations
 
@@ -60,6 +62,7 @@ def parse():
     ap.add_argument("--no-sequence", action="store_true", help="skip the learned-guess sequence leg")
     ap.add_argument("--no-drift", action="store_true", help="skip the drifting-spheres two-phase sequence leg")
     ap.add_argument("--no-fourier", action="store_true", help="skip the fourier3d (FFT direct solve) leg")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the gmres_pfmg implicit-solve leg")
     ap.add_argument("--drift-n", type=int, default=256)
     ap.add_argument("--drift-steps", type=int, default=24)
     ap.add_argument("--seq-n", type=int, default=128)
@@ -229,6 +232,58 @@ def fourier_leg(args, dev, solves=10):
     return res
 
 
+def implicit_leg(args, dev, solves=3):
+    """gmres_pfmg on the nonsymmetric implicit-solve test system at n^3 (the
+    channel grid: x/z periodic, y walls, tanh stretch 2; visc 0.02, CFL 1):
+    setup and solve, plain FP64 against the FP32 V-cycle."""
+    import torch
+
+    from nga2_amd import inputs
+    from nga2_amd._lib import FP64, GMRES_PFMG, MIXED
+    from nga2_amd.config import Config, tanh_stretched, uniform
+    from nga2_amd.linsol import DeviceSolve
+
+    t0 = time.time()
+    n = args.n
+    cfg = Config(uniform(n), tanh_stretched(n), uniform(n), no=1, yper=False, name="implicit_channel")
+    P = inputs.implicit_problem(cfg, visc=0.02)
+    log(f"[bench] generated {P.name} in {time.time() - t0:.1f}s")
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
+    d_x = torch.zeros_like(d_rhs)
+    res = {"workload": f"{P.name}: {n}^3, x/z periodic, y walls tanh stretch 2, rho 1 + dt/2 (central convection "
+                       f"by Gaussian faces |u|max 1 - 0.02 lap), dt = h_min, rcvg {args.rcvg:g}, zero guess; "
+                       "GMRES(5) right-preconditioned by the V-cycle of the symmetric part"}
+    xs = {}
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        ds = DeviceSolve(cfg, method=GMRES_PFMG)
+        ds.setup(d_opr, maxit=200, rcvg=args.rcvg, precision=prec)
+        d_x.zero_()
+        ds.solve(d_rhs, d_x)  # warm-up
+        setup_ms, solve_ms = [], []
+        for _ in range(solves):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            ds.setup(d_opr, maxit=200, rcvg=args.rcvg, precision=prec)
+            torch.cuda.synchronize()
+            setup_ms.append((time.perf_counter() - t) * 1e3)
+            d_x.zero_()
+            t = time.perf_counter()
+            it, _ = ds.solve(d_rhs, d_x)
+            torch.cuda.synchronize()
+            solve_ms.append((time.perf_counter() - t) * 1e3)
+        st = ds.stats()
+        xs[name] = d_x.clone()
+        res[name] = {"iters": it, "restarts": st["restarts"], "solve_ms": round(float(np.median(solve_ms)), 3),
+                     "setup_ms": round(float(np.median(setup_ms)), 3), "true_rerr": st["true_rerr"]}
+        ds.destroy()
+        log(f"[bench] implicit gmres {name}: it={it} solve {res[name]['solve_ms']} ms setup {res[name]['setup_ms']} ms")
+    res["rel_diff_mixed_vs_fp64"] = float(torch.linalg.vector_norm(xs["mixed"] - xs["fp64"]) /
+                                          torch.linalg.vector_norm(xs["fp64"]))
+    res["speedup_solve_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
+    return res
+
+
 def main():
     args = parse()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
@@ -355,6 +410,9 @@ def main():
     fft_res = None
     if not args.no_fourier:
         fft_res = fourier_leg(args, dev)
+    impl_res = None
+    if not args.no_implicit:
+        impl_res = implicit_leg(args, dev)
 
     drift_res = None
     if not args.no_drift:
@@ -396,6 +454,8 @@ def main():
         out["two_phase_drift"] = drift_res
     if fft_res:
         out["fourier3d"] = fft_res
+    if impl_res:
+        out["implicit_gmres"] = impl_res
     print(json.dumps(out), flush=True)
     return 0
 

@@ -173,30 +173,58 @@ __global__ void __launch_bounds__(NT) k_exact_resid(Geo g, Ghost gh, Slots sl, c
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// y = A x with the exact 7-array operator (the product of k_exact_resid)
-__global__ void __launch_bounds__(NT) k_exact_apply(Geo g, Ghost gh, Slots sl, const double *__restrict__ opr,
-                                                    const double *__restrict__ x, double *__restrict__ y) {
+// The exact operator as seven interior arrays (structure of arrays) for the
+// nonsymmetric products of GMRES: o7[s * N + p], stencil slot s in the order
+// centre, +x, -x, +y, -y, +z, -z. Identity rows (centre 0) get 1 and zero
+// couplings; couplings out of a non-periodic box are 0. The same values the
+// AoS products use, read coalesced.
+__global__ void __launch_bounds__(NT) k_soa7(Geo g, Ghost gh, Slots sl, const double *__restrict__ opr,
+                                             double *__restrict__ o7) {
     for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
         int i, j, k;
         decomp0(g, p, i, j, k);
         const double *A = opr + gidx(gh, i, j, k) * sl.nst;
-        double s;
-        double a0 = A[sl.s[0]];
-        if (!(fabs(a0) > 0.0)) {
-            s = x[p];
+        const double a0 = A[sl.s[0]];
+        const bool pt = fabs(a0) > 0.0;
+        o7[p] = pt ? a0 : 1.0;
+        const bool in[6] = {nbp(i, g.n0, g.p0) >= 0, nbm(i, g.n0, g.p0) >= 0, nbp(j, g.n1, g.p1) >= 0,
+                            nbm(j, g.n1, g.p1) >= 0, nbp(k, g.n2, g.p2) >= 0, nbm(k, g.n2, g.p2) >= 0};
+        for (int q = 0; q < 6; q++) o7[(q + 1) * g.N + p] = (pt && in[q]) ? A[sl.s[q + 1]] : 0.0;
+    }
+}
+
+// y = A x, or (b != nullptr) r = b - A x with partial ||r||^2 in part, from the
+// seven-array form; the same sum order as k_exact_resid
+__global__ void __launch_bounds__(NT) k_apply7(Geo g, const double *__restrict__ o7, const double *__restrict__ b,
+                                               const double *__restrict__ x, double *__restrict__ y,
+                                               double *__restrict__ part) {
+    __shared__ double sm[NT / 64];
+    double acc = 0.0;
+    const long long N = g.N;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < N; p += (long long)gridDim.x * NT) {
+        int i, j, k;
+        decomp0(g, p, i, j, k);
+        double s = o7[p] * x[p];
+        const int ip = nbp(i, g.n0, g.p0), im = nbm(i, g.n0, g.p0);
+        const int jp = nbp(j, g.n1, g.p1), jm = nbm(j, g.n1, g.p1);
+        const int kp = nbp(k, g.n2, g.p2), km = nbm(k, g.n2, g.p2);
+        if (ip >= 0) s += o7[N + p] * x[lin0(g, ip, j, k)];
+        if (im >= 0) s += o7[2 * N + p] * x[lin0(g, im, j, k)];
+        if (jp >= 0) s += o7[3 * N + p] * x[lin0(g, i, jp, k)];
+        if (jm >= 0) s += o7[4 * N + p] * x[lin0(g, i, jm, k)];
+        if (kp >= 0) s += o7[5 * N + p] * x[lin0(g, i, j, kp)];
+        if (km >= 0) s += o7[6 * N + p] * x[lin0(g, i, j, km)];
+        if (b) {
+            const double rv = b[p] - s;
+            y[p] = rv;
+            acc += rv * rv;
         } else {
-            s = a0 * x[p];
-            int ip = nbp(i, g.n0, g.p0), im = nbm(i, g.n0, g.p0);
-            int jp = nbp(j, g.n1, g.p1), jm = nbm(j, g.n1, g.p1);
-            int kp = nbp(k, g.n2, g.p2), km = nbm(k, g.n2, g.p2);
-            if (ip >= 0) s += A[sl.s[1]] * x[lin0(g, ip, j, k)];
-            if (im >= 0) s += A[sl.s[2]] * x[lin0(g, im, j, k)];
-            if (jp >= 0) s += A[sl.s[3]] * x[lin0(g, i, jp, k)];
-            if (jm >= 0) s += A[sl.s[4]] * x[lin0(g, i, jm, k)];
-            if (kp >= 0) s += A[sl.s[5]] * x[lin0(g, i, j, kp)];
-            if (km >= 0) s += A[sl.s[6]] * x[lin0(g, i, j, km)];
+            y[p] = s;
         }
-        y[p] = s;
+    }
+    if (part) {
+        const double t = block_sum(acc, sm);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
     }
 }
 
@@ -572,6 +600,7 @@ struct ngap_t {
     // preconditioned directions Z (kdim), and the per-step scalars
     int kdim = 0;
     double *gV = nullptr, *gZ = nullptr;
+    double *o7 = nullptr;    // the exact operator as seven interior arrays
     double *gm = nullptr;    // device scalars: [0,64) dots, [64,128) second-pass dots, 128 ||w||^2, [192,256) y
     double *h_gm = nullptr;  // pinned copy
 };
@@ -669,7 +698,8 @@ void free_all(ngap_t *h) {
     dfree(h, h->gV, sizeof(double) * N * (h->kdim + 1));
     dfree(h, h->gZ, sizeof(double) * N * h->kdim);
     dfree(h, h->gm, sizeof(double) * 256);
-    h->gV = h->gZ = h->gm = nullptr;
+    dfree(h, h->o7, sizeof(double) * N * 7);
+    h->gV = h->gZ = h->gm = h->o7 = nullptr;
     h->kdim = 0;
     dfree(h, h->parts, sizeof(double) * 4 * MAX_PARTS);
     dfree(h, h->scal, sizeof(double) * NSCAL_ALLOC);
@@ -1000,6 +1030,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
             dmalloc(h, (void **)&h->gZ, sizeof(double) * N * h->kdim))
             return NGAP_ERR_ALLOC;
         if (!h->gm && dmalloc(h, (void **)&h->gm, sizeof(double) * 256)) return NGAP_ERR_ALLOC;
+        if (!h->o7 && dmalloc(h, (void **)&h->o7, sizeof(double) * N * 7)) return NGAP_ERR_ALLOC;
         if (!h->h_gm) HIPCHK(hipHostMalloc((void **)&h->h_gm, sizeof(double) * 256, hipHostMallocDefault));
     }
     hipStream_t s = h->stream;
@@ -1020,6 +1051,8 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     }
     Level &L0 = h->lev[0];
     int nb = grid_for(h->g0.N);
+    if (h->method == NGAP_GMRES_PFMG)
+        hipLaunchKernelGGL(k_soa7, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots, (const double *)h->opr, h->o7);
     hipLaunchKernelGGL(k_level0, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots, (const double *)h->opr, L0.d.c,
                        L0.d.a0, L0.d.a1, L0.d.a2, h->part, h->parts, h->parts + MAX_PARTS);
     hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, (int)S_ASYM);
@@ -1454,7 +1487,9 @@ int run_gmres(ngap_t *h) {
     int cycles = 0;
     while (true) {
         // true residual: r = b - A x into V_0
-        exact_residual(h, h->gV, S_RRT);
+        hipLaunchKernelGGL(k_apply7, dim3(nb), dim3(NT), 0, s, h->g0, (const double *)h->o7, (const double *)h->b,
+                           (const double *)h->x, h->gV, h->parts);
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, (int)S_RRT);
         HIPCHK(hipGetLastError());
         if ((rc = read_scal(h))) return rc;
         const double beta = std::sqrt(h->h_scal[S_RRT]);
@@ -1476,8 +1511,8 @@ int run_gmres(ngap_t *h) {
             double *vj = h->gV + (size_t)j * N, *zj = h->gZ + (size_t)j * N;
             double *w = h->gV + (size_t)(j + 1) * N;
             gmres_precond(h, vj, zj, mixed);
-            hipLaunchKernelGGL(k_exact_apply, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots,
-                               (const double *)h->opr, (const double *)zj, w);
+            hipLaunchKernelGGL(k_apply7, dim3(nb), dim3(NT), 0, s, h->g0, (const double *)h->o7,
+                               (const double *)nullptr, (const double *)zj, w, (double *)nullptr);
             cgs_pass(h, w, h->gV, j + 1, h->gm, -1);
             cgs_pass(h, w, h->gV, j + 1, h->gm + 64, 128);
             HIPCHK(hipGetLastError());

@@ -1,5 +1,5 @@
-!> MI355X pressure solver behind NGA2's linsol interface.
-!> Drop-in for hypre_str with method pcg_pfmg (src/solver/hypre_str_class.f90
+!> MI355X structured solver behind NGA2's linsol interface.
+!> Drop-in for hypre_str with methods pcg_pfmg and gmres_pfmg (src/solver/hypre_str_class.f90
 !> in NGA2): same constructor ngap_str(cfg,name,method,nst), same public
 !> fields (opr, rhs, sol, stc, stmap, maxit, rcvg, it, rerr, aerr, maxlevel,
 !> setup_done) and the same init/setup/solve/destroy/log/print/print_short
@@ -24,7 +24,8 @@ module ngap_str_class
    !> ngap_str object definition
    type, extends(linsol) :: ngap_str
       integer :: maxlevel                                 !< Maximum number of multigrid levels (0: no cap)
-      integer :: precision=ngap_fp64                      !< ngap_fp64, or ngap_mixed (FP32 V-cycle under FP64 PCG)
+      integer :: precision=ngap_fp64                      !< ngap_fp64, or ngap_mixed (FP32 V-cycle under FP64 PCG/GMRES)
+      integer :: kdim=5                                   !< GMRES restart length (gmres_pfmg; HYPRE's default 5)
       type(c_ptr), private :: handle=c_null_ptr           !< libngapsolve handle (grid, stencil, device data)
       type(ngap_stats_t) :: stats                         !< statistics of the last solve
    contains
@@ -115,6 +116,7 @@ contains
       if (.not.c_associated(this%handle)) call this%init()
       call check(ngap_default_opts(o),'setup')
       o%precision=int(this%precision,c_int)
+      o%kdim=int(this%kdim,c_int)
       call check(ngap_setup(this%handle,this%opr,int(this%maxit,c_int),real(this%rcvg,c_double),int(this%maxlevel,c_int),o),'setup')
       this%setup_done=.true.
    end subroutine ngap_str_setup

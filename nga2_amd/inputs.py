@@ -336,12 +336,14 @@ WORKLOADS = {
 
 def implicit_problem(cfg, seed=4004, rho=1.0, visc=0.02, dt=None, cfl=1.0):
     """A nonsymmetric implicit-solve test system on cfg (incomp.convection_diffusion):
-    Gaussian face velocities scaled to |u|max = 1, dt from the convective
+    Gaussian face velocities (ghosts synced) scaled to |u|max = 1, dt from the convective
     CFL number (dt = cfl * h_min / |u|max), and a Gaussian right-hand side.
     Seeded with numpy PCG64(seed)."""
     rng = _rng(seed)
     met = incomp.Metrics(cfg)
-    faces = [rng.standard_normal(cfg.shape) for _ in range(3)]
+    faces = [np.asfortranarray(rng.standard_normal(cfg.shape)) for _ in range(3)]
+    for f in faces:
+        cfg.sync(f)  # periodic images, as NGA2's velocity arrays hold them
     umax = max(float(np.abs(f[cfg.interior]).max()) for f in faces)
     faces = [f / umax for f in faces]
     h = min(float(np.min(cfg.dx[cfg.interior[0]])), float(np.min(cfg.dy[cfg.interior[1]])),

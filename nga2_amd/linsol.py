@@ -39,7 +39,8 @@ def grid_of(cfg):
 
 
 class NgapStr:
-    """Drop-in for ``hypre_str`` with method ``pcg_pfmg`` on one MI355X."""
+    """Drop-in for ``hypre_str`` with methods ``pcg_pfmg`` and ``gmres_pfmg``
+    on one MI355X (``opts['kdim']``: GMRES restart length, default 5)."""
 
     def __init__(self, cfg, name, method, nst):
         self.cfg = cfg

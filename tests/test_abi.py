@@ -214,3 +214,35 @@ def test_fourier_exports_and_host_checks():
     assert L.ngap_fft_solve(None, None, None) == _lib.ERR_ARG
     assert "uniform" in _lib.strerror(_lib.ERR_NOT_CIRCULANT)
     assert "near zero" in _lib.strerror(_lib.ERR_SINGULAR)
+
+
+def test_convection_diffusion_operator():
+    """The implicit-solve test operator: with zero velocities each interior
+    row sums to rho (diffusion conserves constants, walls included), and on a
+    uniform grid the operator is symmetric; a velocity field adds couplings
+    that are antisymmetric between neighbours (central convection). Per unit
+    volume, as NGA2's implicit operator, so a stretched grid is not
+    symmetric."""
+    import scipy.sparse as sp
+    from conftest import assemble_csr
+    from nga2_amd.config import tanh_stretched
+
+    for cfg, sym in ((Config(uniform(10), tanh_stretched(8), uniform(6), no=1, yper=False), False),
+                     (Config(uniform(10), uniform(8), uniform(6), no=1, yper=False, zper=False), True)):
+        met = incomp.Metrics(cfg)
+        zero = [np.zeros(cfg.shape)] * 3
+        opr = incomp.convection_diffusion(met, zero, 2.0, 0.1, 0.3)
+        A = assemble_csr(cfg, opr)
+        assert np.allclose(np.asarray(A.sum(axis=1)).ravel(), 2.0, rtol=0, atol=1e-12)
+        asym = abs(A - A.T).max()
+        assert (asym <= 1e-12 * abs(A).max()) == sym
+        if not sym:
+            continue
+        rng = np.random.default_rng(0)
+        faces = [np.asfortranarray(rng.standard_normal(cfg.shape)) for _ in range(3)]
+        for f in faces:
+            cfg.sync(f)
+        C = assemble_csr(cfg, incomp.convection_diffusion(met, faces, 2.0, 0.1, 0.3)) - A
+        Coff = C - sp.diags(C.diagonal())
+        assert abs(Coff + Coff.T).max() <= 1e-12 * abs(Coff).max()
+        assert abs(Coff).max() > 0.0
