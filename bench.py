@@ -433,7 +433,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(1e3 * el / steps, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32", "data": "synthetic",
         "config": {"workload": f"{P.name}: {cfg.nx}x{cfg.ny}x{cfg.nz}, x/z periodic, y walls tanh stretch 2, "
-                               f"spectral RHS seed 2002; FP64 flexible PCG + FP32 PFMG-style V(1,1), "
+                               f"spectral RHS seed 2002; FP64 flexible PCG + FP32 PFMG-style V(2,2), "
                                f"rcvg {args.rcvg:g}, zero guess, inputs resident in HBM",
                    "cells": N, "precision": "mixed"},
         "iters": mixed["its"][0], "true_rerr": mixed["stats"]["true_rerr"], "err_vs_fp64_oracle": err_mixed,

@@ -92,8 +92,8 @@ typedef struct ngap_grid {
  * fixed at :267-272). Fill with ngap_default_opts. */
 typedef struct ngap_opts {
     int precision;    /* NGAP_FP64 (default) or NGAP_MIXED */
-    int nu_pre;       /* pre-smoothing RB sweeps per level, default 1 */
-    int nu_post;      /* post-smoothing BR sweeps per level, default 1 */
+    int nu_pre;       /* pre-smoothing RB sweeps per level; 0 (default): 1 in FP64, 2 in NGAP_MIXED */
+    int nu_post;      /* post-smoothing BR sweeps per level; 0 (default): 1 in FP64, 2 in NGAP_MIXED */
     int coarsest_max; /* stop coarsening at <= this many cells, default 128 */
     int check_finite; /* scan opr/rhs/guess for NaN/Inf, default 1 */
     int verbose;      /* 0 quiet, 1 one line per solve on stderr */

@@ -47,7 +47,7 @@ def test_strerror_and_defaults_without_gpu():
     assert _lib.strerror(0) == "ok"
     assert "setup" in _lib.strerror(_lib.ERR_NOT_SETUP)
     o = _lib.default_opts()
-    assert (o.precision, o.nu_pre, o.nu_post, o.check_finite) == (0, 1, 1, 1)
+    assert (o.precision, o.nu_pre, o.nu_post, o.check_finite, o.kdim) == (0, 0, 0, 1, 5)
 
 
 def test_create_rejects_bad_arguments_without_gpu():

@@ -71,7 +71,8 @@ def case(request):
     print(f"\n[case] {request.param}", flush=True)
     P = CASES[request.param]()
     o = Oracle.from_cfg(P.cfg, P.opr)
-    ps = _solver(P, precision=MIXED)
+    # V(1,1) in both precisions, as the oracle's V-cycle
+    ps = _solver(P, precision=MIXED, nu_pre=1, nu_post=1)
     yield request.param, P, o, ps
     ps.destroy()
 
