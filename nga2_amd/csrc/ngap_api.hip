@@ -895,6 +895,8 @@ void launch_bottom_levels(ngap_t *h, int lb) {
     launch_bottom<T>(h->stream, a);
 }
 
+constexpr long long PAIR_MIN_CELLS = 1LL << 24;  // levels whose post-smoothing pairs are fused
+
 // One V(nu_pre, nu_post) cycle from a zero guess: v0 = M f0.
 // Sweeps are out of place; on each level the iterate alternates between the
 // level's two buffers so that the last post-smoothing sweep lands in the
@@ -905,6 +907,12 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
     const int nl = h->nlev;
     hipStream_t s = h->stream;
     const int npre = h->opts.nu_pre, npost = h->opts.nu_post;
+    // On levels of >= PAIR_MIN_CELLS cells the post-smoothing sweeps run in
+    // fused pairs (one launch each) plus a single one for an odd count; every
+    // launch switches the level's buffer. At 256^3 the pair takes 244 us
+    // against 2 x 131 us on the fine level, but is slower on the smaller
+    // levels, whose chunks are too short to amortise its deeper pipeline.
+    auto post_launches = [&](const Level &L) { return L.g.N >= PAIR_MIN_CELLS ? npost / 2 + (npost & 1) : npost; };
     const int lb = bottom_start(h);
     const int top = lb < nl ? lb : nl - 1;  // first level below the per-level loop
     std::vector<T *> pre(nl);
@@ -913,7 +921,8 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
         T *fin = l == 0 ? v0 : D.v;
-        T *bufs[2] = {((npre + npost) & 1) ? fin : D.w, ((npre + npost) & 1) ? D.w : fin};
+        const int npl = post_launches(L);
+        T *bufs[2] = {((npre + npl) & 1) ? fin : D.w, ((npre + npl) & 1) ? D.w : fin};
         const T *cur = nullptr;
         int nb = 0;
         for (int sw = 0; sw < npre; sw++) {
@@ -948,9 +957,15 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         LevelData<T> &DC = data<T>(Cn);
         T *cur = pre[l];
         launch_prolong_to<T>(s, L.g, Cn.g, L.cdir, DC.pm, DC.pp, DC.v, cur, cur);
-        for (int sw = 0; sw < npost; sw++) {
+        for (int sw = 0; sw < npost;) {
             T *nxt = (cur == fin) ? D.w : fin;
-            launch_sweep<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, nxt, 1);
+            if (npost - sw >= 2 && L.g.N >= PAIR_MIN_CELLS) {
+                launch_sweep2<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, nxt, 1);
+                sw += 2;
+            } else {
+                launch_sweep<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, nxt, 1);
+                sw++;
+            }
             cur = nxt;
         }
     }
@@ -1710,6 +1725,8 @@ static int t_rbgs(ngap_t *h, int l, const double *f, double *v, int order) {
     if (rc || !dout) return rc ? rc : NGAP_ERR_ALLOC;
     if (order & 2)  // first sweep from a zero iterate (the V-cycle's pre-smoothing)
         launch_smooth0<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dout, order & 1);
+    else if (order & 4)  // two sweeps of the same order in one launch (the post-smoothing pair)
+        launch_sweep2<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dout, order & 1);
     else
         launch_sweep<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dout, order & 1);
     HIPCHK(hipGetLastError());

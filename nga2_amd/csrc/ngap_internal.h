@@ -19,6 +19,10 @@ void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
 template <typename T>
 void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
                   const T *vin, T *vout, int c1);
+// two fused sweeps of the same colour order (vin != vout)
+template <typename T>
+void launch_sweep2(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                   const T *vin, T *vout, int c1);
 // first symmetric sweep from a zero iterate as a stencil (no LDS)
 template <typename T>
 void launch_smooth0(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f, T *v,

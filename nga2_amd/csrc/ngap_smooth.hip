@@ -545,6 +545,176 @@ __global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c
     }
 }
 
+// ------------------------------------------------------------------ two fused sweeps
+// Two consecutive sweeps of the same colour order c1 (the post-smoothing of a
+// V(., 2) cycle) in one launch: four colour half-steps
+//   h1: colour c1   from vin       h2: colour c2 from h1
+//   h3: colour c1   from h2        h4: colour c2 from h3
+// vout = h3 on c1 cells, h4 on c2 cells: the result of two k_sweep launches,
+// with the same arithmetic in the same order, so bit-identical to them.
+// A 64 x 16 workgroup recomputes a 3-cell ring (output core 58 x 10): h1 runs
+// on every position (the outer ring reads the cells just outside the tile),
+// h2 from depth 1, h3 from depth 2, h4 on the core. The planes are pipelined:
+// at step s, h1 on plane s+3, h2 on s+2, h3 on s+1, h4 on s. Each field
+// lives in LDS for the in-plane neighbours of the next half-step and in
+// registers (three planes) for its z neighbours; a_x, a_y are in LDS for the
+// four planes in use. One LDS barrier per step.
+constexpr int R2 = 3;  // recompute ring of the fused pair
+
+template <typename T, int SY>
+__global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
+                                                   const T *__restrict__ a1, const T *__restrict__ a2,
+                                                   const T *__restrict__ f, const T *__restrict__ vin,
+                                                   T *__restrict__ vout, int c1, int kc, int gx, int gy) {
+    __shared__ T s_vin[2][SY][SX];
+    __shared__ T s_a0[5][SY][SX];
+    __shared__ T s_a1[5][SY][SX];
+    __shared__ T s_h[3][2][SY][SX];  // fields h1, h2, h3, two planes each
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int plane = n0 * n1;
+    int bx, by, bz;
+    tile_of(gx, gy, bx, by, bz);
+    const int iu = bx * (SX - 2 * R2) - R2 + tx;
+    const int ju = by * (SY - 2 * R2) - R2 + ty;
+    const int ii = wrapc(iu, n0, g.p0);
+    const int jj = wrapc(ju, n1, g.p1);
+    const bool valid = ii >= 0 && jj >= 0;
+    const int col = valid ? jj * n0 + ii : 0;
+    const int depth = min(min(tx, SX - 1 - tx), min(ty, SY - 1 - ty));
+    const bool left = tx == 0, right = tx == SX - 1, bot = ty == 0, top = ty == SY - 1;
+    const bool corner = (left || right) && (bot || top);
+    const bool core = depth >= R2 && iu < n0 && ju < n1;
+    int cout = -1;
+    if (valid && !corner) {
+        if (left) { int q = wrapc(ii - 1, n0, g.p0); cout = q >= 0 ? jj * n0 + q : -1; }
+        if (right) { int q = wrapc(ii + 1, n0, g.p0); cout = q >= 0 ? jj * n0 + q : -1; }
+        if (bot) { int q = wrapc(jj - 1, n1, g.p1); cout = q >= 0 ? q * n0 + ii : -1; }
+        if (top) { int q = wrapc(jj + 1, n1, g.p1); cout = q >= 0 ? q * n0 + ii : -1; }
+    }
+    const T *__restrict__ aout = right ? a0 : a1;
+    const bool out_owns = right || top;
+    const int par = (ii + jj) & 1;
+    const int c2 = c1 ^ 1;
+    const int k0 = bz * kc;
+    const int k1 = min(n2, k0 + kc);
+
+    auto kwrap = [&](int k) { return wrapc(k, n2, g.p2); };
+    auto ld = [&](const T *__restrict__ a, int kk) -> T { return (valid && kk >= 0) ? a[kk * plane + col] : T(0); };
+    auto ldo = [&](const T *__restrict__ a, int kk) -> T { return (cout >= 0 && kk >= 0) ? a[kk * plane + cout] : T(0); };
+    auto ldc = [&](int kk) -> T { return valid && kk >= 0 ? c[kk * plane + col] : T(1); };
+    // colour of this cell on (unwrapped) plane k, from its wrapped index
+    auto colour = [&](int kk) { return (par + kk) & 1; };
+
+    // registers: index 0 = plane P = s+3 (h1's plane), 1 = P-1, 2 = P-2, 3 = P-3
+    T cq[4], fq[4];
+    T a2q[5];        // a2 of planes P+1, P, P-1, P-2, P-3
+    T vin_m, vin_p;  // vin of planes P-1 and P+1
+    T h1[3], h2[3], h3[3];  // h1 of P, P-1, P-2; h2 of P-1, P-2, P-3; h3 of P-2, P-3, P-4
+    // data of plane P for the first step
+    int s = k0 - 6;
+    int kP = kwrap(s + 3);
+    for (int q = 0; q < 4; q++) {
+        const int kq = kwrap(s + 3 - q);
+        cq[q] = ldc(kq);
+        fq[q] = ld(f, kq);
+    }
+    for (int q = 0; q < 5; q++) a2q[q] = ld(a2, kwrap(s + 4 - q));
+    vin_m = ld(vin, kwrap(s + 2));
+    vin_p = ld(vin, kwrap(s + 4));
+    T vin_P = ld(vin, kP), a0_P = ld(a0, kP), a1_P = ld(a1, kP);
+    T vo_P = ldo(vin, kP), ao_P = out_owns ? ldo(aout, kP) : T(0);
+    for (int q = 0; q < 3; q++) h1[q] = h2[q] = h3[q] = T(0);
+    int sa = 0;  // slot of plane P in the 5-deep a0/a1 ring
+    for (; s < k1; s++) {
+        const int P = s + 3;
+        // prefetch plane P+1 (and vin, a2 of P+2)
+        const int kN = kwrap(P + 1), kN2 = kwrap(P + 2);
+        const T c_N = ldc(kN), f_N = ld(f, kN), a0_N = ld(a0, kN), a1_N = ld(a1, kN);
+        const T vo_N = ldo(vin, kN), ao_N = out_owns ? ldo(aout, kN) : T(0);
+        const T vin_N2 = ld(vin, kN2), a2_N2 = ld(a2, kN2);
+        // publish plane P
+        const int sv = P & 1;
+        s_vin[sv][ty][tx] = vin_P;
+        s_a0[sa][ty][tx] = a0_P;
+        s_a1[sa][ty][tx] = a1_P;
+        lds_barrier();
+        const int sa1 = sa == 0 ? 4 : sa - 1, sa2 = sa1 == 0 ? 4 : sa1 - 1, sa3 = sa2 == 0 ? 4 : sa2 - 1;
+        // one half-step at plane p (ring slot sl of a_x, a_y): in-plane neighbours
+        // from field slab F, z neighbours zm / zp, couplings a2 below / above
+        auto half = [&](int q, int sl, const T (*F)[SX], T zm, T zp, T a2m, T a2p, bool ring) -> T {
+            T sum = fq[q];
+            const T a0s = s_a0[sl][ty][tx], a1s = s_a1[sl][ty][tx];
+            const T xm = (ring && left) ? vo_P : F[ty][tx - 1];
+            const T xp = (ring && right) ? vo_P : F[ty][tx + 1];
+            const T ym = (ring && bot) ? vo_P : F[ty - 1][tx];
+            const T yp = (ring && top) ? vo_P : F[ty + 1][tx];
+            const T a0p = (ring && right) ? ao_P : s_a0[sl][ty][tx + 1];
+            const T a1p = (ring && top) ? ao_P : s_a1[sl][ty + 1][tx];
+            sum -= a0s * xm;
+            sum -= a0p * xp;
+            sum -= a1s * ym;
+            sum -= a1p * yp;
+            sum -= a2m * zm;
+            sum -= a2p * zp;
+            return sum / cq[q];
+        };
+        // h1 on P (every position but the corners; the outer ring reads outside values)
+        T n1v = T(0);
+        if (valid && kP >= 0 && !corner && colour(kP) == c1)
+            n1v = half(0, sa, s_vin[sv], vin_m, vin_p, a2q[1], a2q[0], true);
+        // h2 on P-1 from h1 (depth >= 1)
+        const int k_1 = kwrap(P - 1);
+        T n2v = T(0);
+        if (valid && k_1 >= 0 && depth >= 1 && colour(k_1) == c2)
+            n2v = half(1, sa1, s_h[0][(P - 1) & 1], h1[1], n1v, a2q[2], a2q[1], false);
+        // h3 on P-2 from h2 (depth >= 2)
+        const int k_2 = kwrap(P - 2);
+        T n3v = T(0);
+        if (valid && k_2 >= 0 && depth >= 2 && colour(k_2) == c1)
+            n3v = half(2, sa2, s_h[1][(P - 2) & 1], h2[1], n2v, a2q[3], a2q[2], false);
+        // h4 on P-3 = s from h3, stored on the core
+        const int ks = kwrap(s);
+        if (s >= k0 && core && ks >= 0) {
+            T out;
+            if (colour(ks) == c1) {
+                out = h3[0];
+            } else {
+                out = half(3, sa3, s_h[2][s & 1], h3[1], n3v, a2q[4], a2q[3], false);
+            }
+            vout[ks * plane + col] = out;
+        }
+        // publish the new field planes (read at the next step, after its barrier)
+        s_h[0][P & 1][ty][tx] = n1v;
+        s_h[1][(P - 1) & 1][ty][tx] = n2v;
+        s_h[2][(P - 2) & 1][ty][tx] = n3v;
+        // rotate: plane P+1 becomes P
+        h3[1] = h3[0];
+        h3[0] = n3v;
+        h2[1] = h2[0];
+        h2[0] = n2v;
+        h1[1] = h1[0];
+        h1[0] = n1v;
+        for (int q = 3; q > 0; q--) {
+            cq[q] = cq[q - 1];
+            fq[q] = fq[q - 1];
+        }
+        cq[0] = c_N;
+        fq[0] = f_N;
+        for (int q = 4; q > 0; q--) a2q[q] = a2q[q - 1];
+        a2q[0] = a2_N2;
+        vin_m = vin_P;
+        vin_P = vin_p;
+        vin_p = vin_N2;
+        a0_P = a0_N;
+        a1_P = a1_N;
+        vo_P = vo_N;
+        ao_P = ao_N;
+        kP = kN;
+        sa = sa == 4 ? 0 : sa + 1;
+    }
+}
+
 // ------------------------------------------------------------------ restriction
 __device__ __forceinline__ int gdim_(const Geo &g, int d) { return d == 0 ? g.n0 : (d == 1 ? g.n1 : g.n2); }
 __device__ __forceinline__ int gper_(const Geo &g, int d) { return d == 0 ? g.p0 : (d == 1 ? g.p1 : g.p2); }
@@ -1012,6 +1182,25 @@ void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
     launch_sweep_sy<T, 16>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
 }
 
+// Two sweeps of colour order c1 in one launch (k_sweep2): 58 x 10 output
+// tiles, about 512 workgroups, z chunks of >= 16 planes on large levels (each
+// chunk recomputes 6 planes). vin and vout must differ.
+template <typename T>
+void launch_sweep2(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
+                   const T *vin, T *vout, int c1) {
+    constexpr int SY = 16;
+    const int gx = (g.n0 + SX - 2 * R2 - 1) / (SX - 2 * R2), gy = (g.n1 + SY - 2 * R2 - 1) / (SY - 2 * R2);
+    const long long tiles = (long long)gx * gy;
+    long long gz = std::max<long long>(1, 512 / tiles);
+    if (g.N >= (1 << 22) && gz > 1 && (g.n2 + gz - 1) / gz < 16) gz = std::max<long long>(1, g.n2 / 16);
+    if (gz > g.n2) gz = g.n2;
+    int kc = (int)((g.n2 + gz - 1) / gz);
+    if (kc < 1) kc = 1;
+    const int nz = (g.n2 + kc - 1) / kc;
+    hipLaunchKernelGGL((k_sweep2<T, SY>), dim3((unsigned)(gx * gy * nz)), dim3(SX, SY), 0, s, g, c, a0, a1, a2, f,
+                       vin, vout, c1, kc, gx, gy);
+}
+
 template <typename T>
 void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
                        const T *vin, T *vout) {
@@ -1039,6 +1228,8 @@ template void launch_bottom<float>(hipStream_t, const BotArgs<float> &);
                                   T *, double *);                                                                  \
     template void launch_sweep<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
                                   const T *, T *, int);                                                            \
+    template void launch_sweep2<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *,          \
+                                   const T *, const T *, T *, int);                                                \
     template void launch_smooth0<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *,          \
                                     const T *, T *, int);                                                          \
     template void launch_resid<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
