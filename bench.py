@@ -219,7 +219,7 @@ def fourier_leg(args, dev, solves=10):
     ps.destroy()
     N = cfg.nx * cfg.ny * cfg.nz
     res = {"workload": f"{P.name}: {cfg.nx}^3 triply periodic, uniform, Gaussian faces seed 1001, direct FFT solve "
-                       "(rocFFT D2Z/Z2D FP64 + HIP gather/divide/scatter), inputs resident in HBM",
+                       "(rocFFT D2Z/Z2D FP64 + HIP divide, copy and ghost fill), inputs resident in HBM",
            "setup_ms": round(setup_ms, 3), "solve_ms": round(float(np.median(wall)), 3),
            "solve_ms_device": round(float(np.median(dev_ms)), 3),
            "solves_per_s": round(1e3 / float(np.median(wall)), 2),

@@ -2,8 +2,8 @@
 // on a periodic uniform grid (fourier3d_class.f90:1103-1202), with rocFFT
 // (through hipFFT) for the transforms and HIP kernels for everything around
 // them: the circulance check, the placement of the stencil, the inversion of
-// the transformed operator, and the fused gather / multiply / scatter of a
-// solve.
+// the transformed operator, the spectral divide, the copy of the solution into
+// sol and its ghost fill.
 //
 // The transforms are real-to-complex over the directions with more than one
 // cell, slowest first (z, y, x). The forward one reads the interior of the

@@ -37,8 +37,10 @@ constexpr int W_STRIDE = 40;        // bf16 per weight row (co) in LDS
 __global__ void __launch_bounds__(256) k_conv3_fwd(const __bf16 *__restrict__ in, const __bf16 *__restrict__ wt,
                                                    float *__restrict__ out32, __bf16 *__restrict__ outb, int nx,
                                                    int ny, int nz, int cin, int cout, int relu) {
+    // weights are staged one plane of taps (9 of 27) at a time, so the tile
+    // takes 41 + 18 KB of LDS and two workgroups fit on a CU
     __shared__ __attribute__((aligned(16))) __bf16 s_in[HVOX * IN_STRIDE];
-    __shared__ __attribute__((aligned(16))) __bf16 s_w[27 * CK * W_STRIDE];
+    __shared__ __attribute__((aligned(16))) __bf16 s_w[9 * CK * W_STRIDE];
     const int tilesx = nx / TXV, tilesy = ny / TYV;
     const int t = blockIdx.x;
     const int x0 = (t % tilesx) * TXV;
@@ -60,28 +62,30 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(const __bf16 *__restrict__ in
             const uint4 val = *reinterpret_cast<const uint4 *>(in + gv * cin + ci0 + part * 8);
             *reinterpret_cast<uint4 *>(s_in + hv * IN_STRIDE + part * 8) = val;
         }
-        // stage the weights of this ci chunk: 27 taps x 32 co x 32 ci
-        for (int q = tid; q < 27 * CK * 4; q += 256) {
-            const int row = q >> 2, part = q & 3;  // row = tap*32 + co
-            const int tap = row / CK, co = row % CK;
-            const uint4 val =
-                *reinterpret_cast<const uint4 *>(wt + ((long long)tap * cout + co0 + co) * cin + ci0 + part * 8);
-            *reinterpret_cast<uint4 *>(s_w + row * W_STRIDE + part * 8) = val;
-        }
-        __syncthreads();
-        for (int tap = 0; tap < 27; tap++) {
-            const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;  // halo offsets (0..2)
-            bf16x8 bfr[2];
-            for (int nb = 0; nb < 2; nb++)
-                bfr[nb] = *reinterpret_cast<const bf16x8 *>(s_w + (tap * CK + nb * 16 + lrow) * W_STRIDE + lk);
-            for (int mb = 0; mb < TYV; mb++) {
-                const int hv = (lrow + dx) + HX * ((mb + dy) + HY * (wave + dz));
-                const bf16x8 afr = *reinterpret_cast<const bf16x8 *>(s_in + hv * IN_STRIDE + lk);
-                for (int nb = 0; nb < 2; nb++)
-                    acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nb], acc[mb][nb], 0, 0, 0);
+        for (int dz = 0; dz < 3; dz++) {
+            // stage the weights of taps dz*9 .. dz*9+8 of this ci chunk: 9 taps x 32 co x 32 ci
+            for (int q = tid; q < 9 * CK * 4; q += 256) {
+                const int row = q >> 2, part = q & 3;  // row = (tap - 9 dz)*32 + co
+                const int tap = dz * 9 + row / CK, co = row % CK;
+                const uint4 val =
+                    *reinterpret_cast<const uint4 *>(wt + ((long long)tap * cout + co0 + co) * cin + ci0 + part * 8);
+                *reinterpret_cast<uint4 *>(s_w + row * W_STRIDE + part * 8) = val;
             }
+            __syncthreads();
+            for (int t = 0; t < 9; t++) {
+                const int dx = t % 3, dy = t / 3;  // halo offsets (0..2)
+                bf16x8 bfr[2];
+                for (int nb = 0; nb < 2; nb++)
+                    bfr[nb] = *reinterpret_cast<const bf16x8 *>(s_w + (t * CK + nb * 16 + lrow) * W_STRIDE + lk);
+                for (int mb = 0; mb < TYV; mb++) {
+                    const int hv = (lrow + dx) + HX * ((mb + dy) + HY * (wave + dz));
+                    const bf16x8 afr = *reinterpret_cast<const bf16x8 *>(s_in + hv * IN_STRIDE + lk);
+                    for (int nb = 0; nb < 2; nb++)
+                        acc[mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr, bfr[nb], acc[mb][nb], 0, 0, 0);
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
     // epilogue: lane holds D[m = 4*(lane/16)+q][n = lane%16] of each 16x16 block
     for (int mb = 0; mb < TYV; mb++)
@@ -102,12 +106,16 @@ __global__ void __launch_bounds__(256) k_conv3_fwd(const __bf16 *__restrict__ in
 // part[g][27][32][32] (co-major within the block). Four waves share the taps.
 constexpr int KT = TXV * TYV * TZV;  // 256 voxels per tile
 constexpr int DT_STRIDE = KT + 8;    // bf16 per co row of the transposed dout tile
-constexpr int IT_STRIDE = HVOX + 8;  // bf16 per ci row of the transposed input tile
+// The transposed input tile is held once per x shift dx = 0, 1, 2 of the taps:
+// copy dx holds halo columns dx .. dx+15 of every (y, z) halo row, so the 8
+// consecutive x of a B fragment are one aligned 16-byte LDS read.
+constexpr int WVOX = TXV * HY * HZ;  // 576 voxels of one shifted copy
+constexpr int IT_STRIDE = WVOX + 8;  // bf16 per ci row of a shifted copy
 
 __global__ void __launch_bounds__(256) k_conv3_wgrad(const __bf16 *__restrict__ in, const __bf16 *__restrict__ dout,
                                                      float *__restrict__ part, int nx, int ny, int nz, int cin,
                                                      int cout, int G) {
-    __shared__ __attribute__((aligned(16))) __bf16 s_inT[CK * IT_STRIDE];
+    __shared__ __attribute__((aligned(16))) __bf16 s_inT[3 * CK * IT_STRIDE];
     __shared__ __attribute__((aligned(16))) __bf16 s_dT[CK * DT_STRIDE];
     const int tilesx = nx / TXV, tilesy = ny / TYV, tilesz = nz / TZV;
     const int ntiles = tilesx * tilesy * tilesz;
@@ -123,20 +131,28 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const __bf16 *__restrict__ 
         const int x0 = (t % tilesx) * TXV;
         const int y0 = ((t / tilesx) % tilesy) * TYV;
         const int z0 = (t / (tilesx * tilesy)) * TZV;
-        // transposed input halo tile: s_inT[ci][hv]
-        for (int q = tid; q < HVOX * CK; q += 256) {
-            const int hv = q / CK, c = q % CK;
+        // transposed input halo tile in its three x-shifted copies:
+        // s_inT[dx][ci][w], w = (hx - dx) + TXV * (hy + HY * hz) for hx - dx in [0, TXV)
+        for (int q = tid; q < HVOX * 4; q += 256) {
+            const int hv = q >> 2, part = q & 3;
             const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
             const int gx = wrapi(x0 - 1 + hx, nx), gy = wrapi(y0 - 1 + hy, ny), gz = wrapi(z0 - 1 + hz, nz);
             const long long gv = (long long)gx + (long long)nx * (gy + (long long)ny * gz);
-            s_inT[c * IT_STRIDE + hv] = in[gv * cin + ci0 + c];
+            const bf16x8 val = *reinterpret_cast<const bf16x8 *>(in + gv * cin + ci0 + part * 8);
+            for (int dx = 0; dx < 3; dx++) {
+                const int wx = hx - dx;
+                if (wx < 0 || wx >= TXV) continue;
+                const int w = wx + TXV * (hy + HY * hz);
+                for (int e = 0; e < 8; e++) s_inT[(dx * CK + part * 8 + e) * IT_STRIDE + w] = val[e];
+            }
         }
         // transposed dout tile: s_dT[co][v]
-        for (int q = tid; q < KT * CK; q += 256) {
-            const int v = q / CK, c = q % CK;
+        for (int q = tid; q < KT * 4; q += 256) {
+            const int v = q >> 2, part = q & 3;
             const int vx = x0 + v % TXV, vy = y0 + (v / TXV) % TYV, vz = z0 + v / (TXV * TYV);
             const long long gv = (long long)vx + (long long)nx * (vy + (long long)ny * vz);
-            s_dT[c * DT_STRIDE + v] = dout[gv * cout + co0 + c];
+            const bf16x8 val = *reinterpret_cast<const bf16x8 *>(dout + gv * cout + co0 + part * 8);
+            for (int e = 0; e < 8; e++) s_dT[(part * 8 + e) * DT_STRIDE + v] = val[e];
         }
         __syncthreads();
         for (int kc = 0; kc < KT / 32; kc++) {
@@ -150,11 +166,10 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const __bf16 *__restrict__ 
                 const int tap = wave + 4 * tt;
                 if (tap >= 27) break;
                 const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
-                const int hv = (vx + dx) + HX * ((vy + dy) + HY * (vz + dz));
+                const int w = vx + TXV * ((vy + dy) + HY * (vz + dz));  // vx is 0 or 8
                 for (int nb = 0; nb < 2; nb++) {
-                    const __bf16 *src = s_inT + (nb * 16 + lrow) * IT_STRIDE + hv;
-                    bf16x8 bfr;
-                    for (int e = 0; e < 8; e++) bfr[e] = src[e];
+                    const bf16x8 bfr =
+                        *reinterpret_cast<const bf16x8 *>(s_inT + (dx * CK + nb * 16 + lrow) * IT_STRIDE + w);
                     for (int mb = 0; mb < 2; mb++)
                         acc[tt][mb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(afr[mb], bfr, acc[tt][mb][nb], 0, 0, 0);
                 }
@@ -222,6 +237,50 @@ __global__ void k_conv3_small_fwd(const float *__restrict__ in, const float *__r
     }
     if (out32) out32[i] = s;
     if (outb) outb[i] = (__bf16)(relu ? fmaxf(s, 0.f) : s);
+}
+
+// The first layer (one input channel): one thread per voxel computes all COUT
+// outputs from its 27 input values, with the weights in LDS (a broadcast read
+// per tap and channel), and writes the voxel's COUT channels as 16-byte
+// stores. Same summation order per output as k_conv3_small_fwd (taps in
+// order). The per-(voxel, channel) form re-read the 27 inputs for every
+// channel and spent its time in index arithmetic.
+template <int COUT>
+__global__ void __launch_bounds__(256) k_conv3_c1_fwd(const float *__restrict__ in, const float *__restrict__ w,
+                                                      float *__restrict__ out32, __bf16 *__restrict__ outb, int nx,
+                                                      int ny, int nz, int relu) {
+    __shared__ float sw[27 * COUT];
+    for (int q = threadIdx.x; q < 27 * COUT; q += 256) sw[q] = w[q];
+    __syncthreads();
+    const int nv = nx * ny * nz;
+    const int v = blockIdx.x * 256 + threadIdx.x;
+    if (v >= nv) return;
+    const int x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
+    float acc[COUT];
+#pragma unroll
+    for (int co = 0; co < COUT; co++) acc[co] = 0.f;
+#pragma unroll
+    for (int tap = 0; tap < 27; tap++) {
+        const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+        const float iv = in[wrapi(x + dx, nx) + nx * (wrapi(y + dy, ny) + ny * wrapi(z + dz, nz))];
+#pragma unroll
+        for (int co = 0; co < COUT; co++) acc[co] += sw[tap * COUT + co] * iv;
+    }
+    const long long o = (long long)v * COUT;
+    if (out32) {
+#pragma unroll
+        for (int co = 0; co < COUT; co += 4)
+            *reinterpret_cast<float4 *>(out32 + o + co) = make_float4(acc[co], acc[co + 1], acc[co + 2], acc[co + 3]);
+    }
+    if (outb) {
+#pragma unroll
+        for (int co = 0; co < COUT; co += 8) {
+            bf16x8 b;
+#pragma unroll
+            for (int q = 0; q < 8; q++) b[q] = (__bf16)(relu ? fmaxf(acc[co + q], 0.f) : acc[co + q]);
+            *reinterpret_cast<bf16x8 *>(outb + o + co) = b;
+        }
+    }
 }
 
 // dW of the small layer: one block per weight, fixed-order block reduction
@@ -432,6 +491,26 @@ __global__ void k_chan_copy(const T *__restrict__ src, T *__restrict__ dst, long
 }
 
 // y[v] = sum_c w[c] x[v][c] (1x1 output layer, x bf16)
+// y[v] = sum_ch w[ch] x[v][ch] for C channels: one thread per voxel reading its
+// C bf16 values as 16-byte loads (C a multiple of 8), channels summed in order
+template <int C>
+__global__ void __launch_bounds__(256) k_conv1_fwd_t(const __bf16 *__restrict__ x, const float *__restrict__ w,
+                                                     float *__restrict__ y, long long nv) {
+    float wr[C];
+#pragma unroll
+    for (int ch = 0; ch < C; ch++) wr[ch] = w[ch];
+    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
+        float s = 0.f;
+#pragma unroll
+        for (int c8 = 0; c8 < C; c8 += 8) {
+            const bf16x8 b = *reinterpret_cast<const bf16x8 *>(x + v * C + c8);
+#pragma unroll
+            for (int q = 0; q < 8; q++) s += wr[c8 + q] * (float)b[q];
+        }
+        y[v] = s;
+    }
+}
+
 __global__ void k_conv1_fwd(const __bf16 *__restrict__ x, const float *__restrict__ w, float *__restrict__ y,
                             long long nv, int c) {
     for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
@@ -535,7 +614,13 @@ int ngapnet_flip_transpose(const void *wt, void *wd, int cin, int cout, void *st
 
 int ngapnet_conv3_small_fwd(const float *in, const float *w, float *out32, void *outb, int nx, int ny, int nz,
                             int cin, int cout, int relu, void *stream) {
-    const long long n = (long long)nx * ny * nz * cout;
+    const long long nv = (long long)nx * ny * nz;
+    if (cin == 1 && cout == 32 && nv < (1LL << 31)) {
+        hipLaunchKernelGGL(k_conv3_c1_fwd<32>, dim3((unsigned)((nv + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in,
+                           w, out32, (__bf16 *)outb, nx, ny, nz, relu);
+        return hipGetLastError() == hipSuccess ? 0 : 1;
+    }
+    const long long n = nv * cout;
     hipLaunchKernelGGL(k_conv3_small_fwd, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, in, w,
                        out32, (__bf16 *)outb, nx, ny, nz, cin, cout, relu);
     return hipGetLastError() == hipSuccess ? 0 : 1;
@@ -607,8 +692,12 @@ int ngapnet_chan_copy_f32(const float *src, float *dst, long long nv, int c, int
 }
 
 int ngapnet_conv1_fwd(const void *x, const float *w, float *y, long long nv, int c, void *stream) {
-    hipLaunchKernelGGL(k_conv1_fwd, dim3(blocks_for(nv)), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, y,
-                       nv, c);
+    if (c == 32)
+        hipLaunchKernelGGL(k_conv1_fwd_t<32>, dim3(blocks_for(nv)), dim3(256), 0, (hipStream_t)stream,
+                           (const __bf16 *)x, w, y, nv);
+    else
+        hipLaunchKernelGGL(k_conv1_fwd, dim3(blocks_for(nv)), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w,
+                           y, nv, c);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -610,7 +610,9 @@ __global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ 
     T cq[4], fq[4];
     T a2q[5];        // a2 of planes P+1, P, P-1, P-2, P-3
     T vin_m, vin_p;  // vin of planes P-1 and P+1
-    T h1[3], h2[3], h3[3];  // h1 of P, P-1, P-2; h2 of P-1, P-2, P-3; h3 of P-2, P-3, P-4
+    // the fields' values on the two planes below the one computed this step:
+    // h1 of P-1, P-2; h2 of P-2, P-3; h3 of P-3, P-4
+    T h1[2], h2[2], h3[2];
     // data of plane P for the first step
     int s = k0 - 6;
     int kP = kwrap(s + 3);
@@ -624,7 +626,7 @@ __global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ 
     vin_p = ld(vin, kwrap(s + 4));
     T vin_P = ld(vin, kP), a0_P = ld(a0, kP), a1_P = ld(a1, kP);
     T vo_P = ldo(vin, kP), ao_P = out_owns ? ldo(aout, kP) : T(0);
-    for (int q = 0; q < 3; q++) h1[q] = h2[q] = h3[q] = T(0);
+    for (int q = 0; q < 2; q++) h1[q] = h2[q] = h3[q] = T(0);
     int sa = 0;  // slot of plane P in the 5-deep a0/a1 ring
     for (; s < k1; s++) {
         const int P = s + 3;
