@@ -304,65 +304,89 @@ __global__ void __launch_bounds__(256) k_conv3_small_wgrad(const float *__restri
     if (threadIdx.x == 0) dw[wi] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
-// dW of a 1-input-channel layer, one pass per row of taps: workgroup (b, row)
-// with row = (dz+1)*3 + (dy+1) accumulates, for its voxels, the three taps
-// dx = -1, 0, 1 of that row for all COUT channels (3*COUT registers), reading
-// each voxel's dout row once as COUT/4 16-byte loads. Partials per workgroup,
-// summed in fixed order by k_small_wgrad_fin.
-constexpr int SW_BLOCKS = 256;
+// dW of a 1-input-channel layer over bricks of 32 x 8 x 8 voxels: the brick's
+// input with a one-voxel halo sits in LDS, thread (voxel column vx, channel
+// quad q) accumulates all 27 taps of its 4 channels (108 registers), so dout
+// is read once, as one 16-byte load per thread and voxel. Partial bricks at
+// the upper faces skip the voxels beyond the grid. Partials per workgroup,
+// part[b][tap][co], summed in fixed order by k_colsum.
+constexpr int SW_BLOCKS = 1024;
+constexpr int SB_X = 32, SB_Y = 8, SB_Z = 8;
+constexpr int SB_HX = SB_X + 2, SB_HY = SB_Y + 2, SB_HZ = SB_Z + 2;
 
 template <int COUT>
-__global__ void __launch_bounds__(256) k_small_wgrad_rows(const float *__restrict__ in, const float *__restrict__ dout,
-                                                          float *__restrict__ part, int nx, int ny, int nz) {
-    __shared__ float sm[4][3 * COUT];
-    const int row = blockIdx.y;
-    const int dz = row / 3 - 1, dy = row % 3 - 1;
-    const long long nv = (long long)nx * ny * nz;
-    float acc[3][COUT];
+__global__ void __launch_bounds__(256) k_small_wgrad_brick(const float *__restrict__ in, const float *__restrict__ dout,
+                                                           float *__restrict__ part, int nx, int ny, int nz) {
+    constexpr int NQ = COUT / 4;
+    static_assert(256 / NQ == SB_X, "one voxel column per thread group");
+    __shared__ float s_in[SB_HX * SB_HY * SB_HZ];
+    __shared__ float s_red[4][27 * COUT];
+    const int tid = threadIdx.x, q = tid % NQ, vx = tid / NQ;
+    const int nbx = (nx + SB_X - 1) / SB_X, nby = (ny + SB_Y - 1) / SB_Y, nbz = (nz + SB_Z - 1) / SB_Z;
+    const int nb = nbx * nby * nbz;
+    float acc[27][4];
 #pragma unroll
-    for (int e = 0; e < 3; e++)
+    for (int t = 0; t < 27; t++)
 #pragma unroll
-        for (int c = 0; c < COUT; c++) acc[e][c] = 0.f;
-    for (long long v = blockIdx.x * 256LL + threadIdx.x; v < nv; v += (long long)gridDim.x * 256) {
-        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
-        const long long base = (long long)nx * (wrapi(y + dy, ny) + (long long)ny * wrapi(z + dz, nz));
-        const float xm = in[base + wrapi(x - 1, nx)], x0 = in[base + x], xp = in[base + wrapi(x + 1, nx)];
-        const float4 *drow = reinterpret_cast<const float4 *>(dout + v * COUT);
-#pragma unroll
-        for (int q = 0; q < COUT / 4; q++) {
-            const float4 d = drow[q];
-            const float dd[4] = {d.x, d.y, d.z, d.w};
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                acc[0][4 * q + k] += dd[k] * xm;
-                acc[1][4 * q + k] += dd[k] * x0;
-                acc[2][4 * q + k] += dd[k] * xp;
-            }
+        for (int k = 0; k < 4; k++) acc[t][k] = 0.f;
+    for (int brick = blockIdx.x; brick < nb; brick += gridDim.x) {
+        const int x0 = (brick % nbx) * SB_X, y0 = ((brick / nbx) % nby) * SB_Y, z0 = (brick / (nbx * nby)) * SB_Z;
+        __syncthreads();
+        // halo coordinates beyond n are clamped to n: only voxels inside the
+        // grid are accumulated, and they read at most one past the last
+        for (int i = tid; i < SB_HX * SB_HY * SB_HZ; i += 256) {
+            const int hx = i % SB_HX, hy = (i / SB_HX) % SB_HY, hz = i / (SB_HX * SB_HY);
+            const int gx = wrapi(min(x0 - 1 + hx, nx), nx), gy = wrapi(min(y0 - 1 + hy, ny), ny),
+                      gz = wrapi(min(z0 - 1 + hz, nz), nz);
+            s_in[i] = in[gx + (long long)nx * (gy + (long long)ny * gz)];
         }
+        __syncthreads();
+        if (x0 + vx >= nx) continue;
+        const int ly1 = min(SB_Y, ny - y0), lz1 = min(SB_Z, nz - z0);
+        for (int lz = 0; lz < lz1; lz++)
+            for (int ly = 0; ly < ly1; ly++) {
+                const long long v = (x0 + vx) + (long long)nx * ((y0 + ly) + (long long)ny * (z0 + lz));
+                const float4 d = reinterpret_cast<const float4 *>(dout + v * COUT)[q];
+#pragma unroll
+                for (int t = 0; t < 27; t++) {
+                    const float x = s_in[(vx + t % 3) + SB_HX * ((ly + (t / 3) % 3) + SB_HY * (lz + t / 9))];
+                    acc[t][0] += d.x * x;
+                    acc[t][1] += d.y * x;
+                    acc[t][2] += d.z * x;
+                    acc[t][3] += d.w * x;
+                }
+            }
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // lanes with the same channel quad: lane % NQ; sum over the wave, then the 4 waves
+    const int lane = tid & 63, wave = tid >> 6;
 #pragma unroll
-    for (int e = 0; e < 3; e++)
+    for (int t = 0; t < 27; t++)
 #pragma unroll
-        for (int c = 0; c < COUT; c++) {
-            float t = acc[e][c];
-            for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
-            if (lane == 0) sm[wave][e * COUT + c] = t;
+        for (int k = 0; k < 4; k++) {
+            float r = acc[t][k];
+            for (int off = NQ; off < 64; off <<= 1) r += __shfl_xor(r, off, 64);
+            if (lane < NQ) s_red[wave][t * COUT + 4 * q + k] = r;
         }
     __syncthreads();
-    for (int i = threadIdx.x; i < 3 * COUT; i += 256)
-        part[((long long)blockIdx.x * 9 + row) * 3 * COUT + i] = sm[0][i] + sm[1][i] + sm[2][i] + sm[3][i];
+    for (int i = tid; i < 27 * COUT; i += 256)
+        part[(long long)blockIdx.x * 27 * COUT + i] = s_red[0][i] + s_red[1][i] + s_red[2][i] + s_red[3][i];
 }
 
-// dw[tap][co] = sum_b part[b][tap / 3][tap % 3][co]
-__global__ void k_small_wgrad_fin(const float *__restrict__ part, float *__restrict__ dw, int nb, int cout) {
-    const int n = 27 * cout;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const int tap = i / cout, co = i % cout;
-        const int row = tap / 3, e = tap % 3;
-        float s = 0.f;
-        for (int b = 0; b < nb; b++) s += part[((long long)b * 9 + row) * 3 * cout + e * cout + co];
-        dw[i] = s;
+// out[i] = sum_b part[b][i], i < n: workgroup of 16 columns x 16 slices, slice
+// s sums b = s, s + 16, ... and the 16 slice sums are added in slice order
+__global__ void __launch_bounds__(256) k_colsum(const float *__restrict__ part, float *__restrict__ out, int nb, int n) {
+    __shared__ float sm[16][17];
+    const int c = threadIdx.x & 15, sl = threadIdx.x >> 4;
+    const int i = blockIdx.x * 16 + c;
+    float s = 0.f;
+    if (i < n)
+        for (int b = sl; b < nb; b += 16) s += part[(long long)b * n + i];
+    sm[sl][c] = s;
+    __syncthreads();
+    if (sl == 0 && i < n) {
+        float t = 0.f;
+        for (int k = 0; k < 16; k++) t += sm[k][c];
+        out[i] = t;
     }
 }
 
@@ -409,6 +433,27 @@ __global__ void k_relu_bwd(const float *__restrict__ dy, const __bf16 *__restric
     }
 }
 
+// The same in pieces of 4 (n % 4 == 0, dy/dx 16-byte and y/dxb 8-byte aligned)
+__global__ void k_relu_bwd_v4(const float *__restrict__ dy, const __bf16 *__restrict__ y, float *__restrict__ dx,
+                              __bf16 *__restrict__ dxb, long long n4) {
+    typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n4; i += (long long)gridDim.x * 256) {
+        const float4 d = reinterpret_cast<const float4 *>(dy)[i];
+        const bf16x4 yv = reinterpret_cast<const bf16x4 *>(y)[i];
+        const float4 g = make_float4((float)yv[0] > 0.f ? d.x : 0.f, (float)yv[1] > 0.f ? d.y : 0.f,
+                                     (float)yv[2] > 0.f ? d.z : 0.f, (float)yv[3] > 0.f ? d.w : 0.f);
+        if (dx) reinterpret_cast<float4 *>(dx)[i] = g;
+        if (dxb) {
+            bf16x4 b;
+            b[0] = (__bf16)g.x;
+            b[1] = (__bf16)g.y;
+            b[2] = (__bf16)g.z;
+            b[3] = (__bf16)g.w;
+            reinterpret_cast<bf16x4 *>(dxb)[i] = b;
+        }
+    }
+}
+
 __global__ void k_to_bf16(const float *__restrict__ s, __bf16 *__restrict__ d, long long n) {
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) d[i] = (__bf16)s[i];
 }
@@ -445,6 +490,28 @@ __global__ void k_pool2_bwd(const float *__restrict__ dout, float *__restrict__ 
     }
 }
 
+// The same in pieces of 4 channels (c % 4 == 0)
+__global__ void k_pool2_bwd_v4(const float *__restrict__ dout, float *__restrict__ din, int nx, int ny, int nz, int c,
+                               int acc) {
+    const int c4 = c / 4;
+    const long long n = (long long)nx * ny * nz * c4;
+    const int mx = nx / 2, my = ny / 2;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int g4 = (int)(i % c4);
+        const long long v = i / c4;
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        const long long u = (x / 2) + (long long)mx * ((y / 2) + (long long)my * (z / 2));
+        const float4 d = *reinterpret_cast<const float4 *>(dout + u * c + 4 * g4);
+        float4 g = make_float4(0.125f * d.x, 0.125f * d.y, 0.125f * d.z, 0.125f * d.w);
+        float4 *o = reinterpret_cast<float4 *>(din) + i;
+        if (acc) {
+            const float4 p = *o;
+            g = make_float4(p.x + g.x, p.y + g.y, p.z + g.z, p.w + g.w);
+        }
+        *o = g;
+    }
+}
+
 // out[fine][c0 + ch] = in[coarse][ch] (nearest-neighbour upsampling into a
 // channel slice of a concatenation buffer with ctot channels)
 __global__ void k_up2(const __bf16 *__restrict__ in, __bf16 *__restrict__ out, int nx, int ny, int nz, int c, int ctot,
@@ -457,6 +524,21 @@ __global__ void k_up2(const __bf16 *__restrict__ in, __bf16 *__restrict__ out, i
         const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
         const long long u = (x / 2) + (long long)mx * ((y / 2) + (long long)my * (z / 2));
         out[v * ctot + c0 + ch] = in[u * c + ch];
+    }
+}
+
+// The same copy in 16-byte pieces of 8 channels (c, ctot, c0 multiples of 8)
+__global__ void k_up2_v8(const __bf16 *__restrict__ in, __bf16 *__restrict__ out, int nx, int ny, int nz, int c,
+                         int ctot, int c0) {
+    const int c8 = c / 8;
+    const long long n = (long long)nx * ny * nz * c8;
+    const int mx = nx / 2, my = ny / 2;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int g = (int)(i % c8);
+        const long long v = i / c8;
+        const int x = (int)(v % nx), y = (int)((v / nx) % ny), z = (int)(v / ((long long)nx * ny));
+        const long long u = (x / 2) + (long long)mx * ((y / 2) + (long long)my * (z / 2));
+        *reinterpret_cast<bf16x8 *>(out + v * ctot + c0 + 8 * g) = *reinterpret_cast<const bf16x8 *>(in + u * c + 8 * g);
     }
 }
 
@@ -478,7 +560,43 @@ __global__ void k_up2_bwd(const float *__restrict__ dout, float *__restrict__ di
     }
 }
 
+// The same in pieces of 4 channels (c, ctot, c0 multiples of 4)
+__global__ void k_up2_bwd_v4(const float *__restrict__ dout, float *__restrict__ din, int nx, int ny, int nz, int c,
+                             int ctot, int c0) {
+    const int mx = nx / 2, my = ny / 2, mz = nz / 2, c4 = c / 4;
+    const long long n = (long long)mx * my * mz * c4;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int g4 = (int)(i % c4);
+        const long long v = i / c4;
+        const int x = (int)(v % mx), y = (int)((v / mx) % my), z = (int)(v / ((long long)mx * my));
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int q = 0; q < 8; q++) {
+            const long long u = (2 * x + (q & 1)) + (long long)nx * ((2 * y + ((q >> 1) & 1)) + (long long)ny * (2 * z + (q >> 2)));
+            const float4 d = *reinterpret_cast<const float4 *>(dout + u * ctot + c0 + 4 * g4);
+            s.x += d.x;
+            s.y += d.y;
+            s.z += d.z;
+            s.w += d.w;
+        }
+        reinterpret_cast<float4 *>(din)[i] = s;
+    }
+}
+
 // copy a channel slice: dst[v][d0 + ch] = src[v][s0 + ch], ch < c (bf16 or fp32)
+// the same copy in 16-byte pieces (c, cs, s0, cd, d0 multiples of 16 / sizeof(T))
+template <typename T>
+__global__ void k_chan_copy_v16(const T *__restrict__ src, T *__restrict__ dst, long long nv, int c, int cs, int s0,
+                                int cd, int d0) {
+    constexpr int E = 16 / sizeof(T);
+    const int ce = c / E;
+    const long long n = nv * ce;
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const int g = (int)(i % ce);
+        const long long v = i / ce;
+        *reinterpret_cast<uint4 *>(dst + v * cd + d0 + g * E) = *reinterpret_cast<const uint4 *>(src + v * cs + s0 + g * E);
+    }
+}
+
 template <typename T>
 __global__ void k_chan_copy(const T *__restrict__ src, T *__restrict__ dst, long long nv, int c, int cs, int s0, int cd,
                             int d0) {
@@ -543,14 +661,6 @@ __global__ void __launch_bounds__(256) k_conv1_bwd(const __bf16 *__restrict__ x,
     if (threadIdx.x < c) part[blockIdx.x * c + threadIdx.x] = sm[0][threadIdx.x] + sm[1][threadIdx.x] + sm[2][threadIdx.x] + sm[3][threadIdx.x];
 }
 
-__global__ void k_sum_parts(const float *__restrict__ part, float *__restrict__ out, int nb, int c) {
-    const int ch = threadIdx.x;
-    if (ch >= c) return;
-    float s = 0.f;
-    for (int b = 0; b < nb; b++) s += part[b * c + ch];
-    out[ch] = s;
-}
-
 // Adam on FP32 master weights; writes the BF16 copy
 __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
                        __bf16 *__restrict__ wb, long long n, float lr, float b1, float b2, float eps, float c1,
@@ -571,6 +681,18 @@ int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     if (b > 16384) b = 16384;
     return (int)(b < 1 ? 1 : b);
+}
+
+template <typename T>
+int chan_copy(const T *src, T *dst, long long nv, int c, int cs, int s0, int cd, int d0, void *stream) {
+    constexpr int E = 16 / sizeof(T);
+    if ((c | cs | s0 | cd | d0) % E == 0 && ((uintptr_t)src | (uintptr_t)dst) % 16 == 0)
+        hipLaunchKernelGGL(k_chan_copy_v16<T>, dim3(blocks_for(nv * c / E)), dim3(256), 0, (hipStream_t)stream, src,
+                           dst, nv, c, cs, s0, cd, d0);
+    else
+        hipLaunchKernelGGL(k_chan_copy<T>, dim3(blocks_for(nv * c)), dim3(256), 0, (hipStream_t)stream, src, dst, nv,
+                           c, cs, s0, cd, d0);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 }  // namespace
@@ -628,11 +750,13 @@ int ngapnet_conv3_small_fwd(const float *in, const float *w, float *out32, void 
 
 int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, float *part, int nx, int ny, int nz,
                               int cin, int cout, void *stream) {
-    if (part && cin == 1 && cout == 32) {
-        hipLaunchKernelGGL(k_small_wgrad_rows<32>, dim3(SW_BLOCKS, 9), dim3(256), 0, (hipStream_t)stream, in, dout, part,
-                           nx, ny, nz);
-        hipLaunchKernelGGL(k_small_wgrad_fin, dim3(blocks_for(27LL * cout)), dim3(256), 0, (hipStream_t)stream,
-                           (const float *)part, dw, SW_BLOCKS, cout);
+    if (part && cin == 1 && cout == 32 && (uintptr_t)dout % 16 == 0) {
+        const long long bricks = (long long)((nx + SB_X - 1) / SB_X) * ((ny + SB_Y - 1) / SB_Y) * ((nz + SB_Z - 1) / SB_Z);
+        const int grid = (int)(bricks < SW_BLOCKS ? bricks : SW_BLOCKS);
+        hipLaunchKernelGGL(k_small_wgrad_brick<32>, dim3(grid), dim3(256), 0, (hipStream_t)stream, in, dout, part, nx,
+                           ny, nz);
+        hipLaunchKernelGGL(k_colsum, dim3((27 * cout + 15) / 16), dim3(256), 0, (hipStream_t)stream,
+                           (const float *)part, dw, grid, 27 * cout);
     } else {
         hipLaunchKernelGGL(k_conv3_small_wgrad, dim3(27 * cin * cout), dim3(256), 0, (hipStream_t)stream, in, dout, dw,
                            nx, ny, nz, cin, cout);
@@ -643,8 +767,13 @@ int ngapnet_conv3_small_wgrad(const float *in, const float *dout, float *dw, flo
 long long ngapnet_small_wgrad_part_size(int cout) { return (long long)SW_BLOCKS * 27 * cout; }
 
 int ngapnet_relu_bwd(const float *dy, const void *y, float *dx, void *dxb, long long n, void *stream) {
-    hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, dy, (const __bf16 *)y, dx,
-                       (__bf16 *)dxb, n);
+    const bool v4 = n % 4 == 0 && ((uintptr_t)dy | (uintptr_t)dx) % 16 == 0 && ((uintptr_t)y | (uintptr_t)dxb) % 8 == 0;
+    if (v4)
+        hipLaunchKernelGGL(k_relu_bwd_v4, dim3(blocks_for(n / 4)), dim3(256), 0, (hipStream_t)stream, dy,
+                           (const __bf16 *)y, dx, (__bf16 *)dxb, n / 4);
+    else
+        hipLaunchKernelGGL(k_relu_bwd, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)stream, dy, (const __bf16 *)y,
+                           dx, (__bf16 *)dxb, n);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -660,35 +789,43 @@ int ngapnet_pool2(const void *in, void *out, int nx, int ny, int nz, int c, void
 }
 
 int ngapnet_pool2_bwd(const float *dout, float *din, int nx, int ny, int nz, int c, int acc, void *stream) {
-    hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for((long long)nx * ny * nz * c)), dim3(256), 0, (hipStream_t)stream,
-                       dout, din, nx, ny, nz, c, acc);
+    if (c % 4 == 0 && ((uintptr_t)dout | (uintptr_t)din) % 16 == 0)
+        hipLaunchKernelGGL(k_pool2_bwd_v4, dim3(blocks_for((long long)nx * ny * nz * c / 4)), dim3(256), 0,
+                           (hipStream_t)stream, dout, din, nx, ny, nz, c, acc);
+    else
+        hipLaunchKernelGGL(k_pool2_bwd, dim3(blocks_for((long long)nx * ny * nz * c)), dim3(256), 0,
+                           (hipStream_t)stream, dout, din, nx, ny, nz, c, acc);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int ngapnet_up2(const void *in, void *out, int nx, int ny, int nz, int c, int ctot, int c0, void *stream) {
-    hipLaunchKernelGGL(k_up2, dim3(blocks_for((long long)nx * ny * nz * c)), dim3(256), 0, (hipStream_t)stream,
-                       (const __bf16 *)in, (__bf16 *)out, nx, ny, nz, c, ctot, c0);
+    if (c % 8 == 0 && ctot % 8 == 0 && c0 % 8 == 0 && ((uintptr_t)in | (uintptr_t)out) % 16 == 0)
+        hipLaunchKernelGGL(k_up2_v8, dim3(blocks_for((long long)nx * ny * nz * c / 8)), dim3(256), 0,
+                           (hipStream_t)stream, (const __bf16 *)in, (__bf16 *)out, nx, ny, nz, c, ctot, c0);
+    else
+        hipLaunchKernelGGL(k_up2, dim3(blocks_for((long long)nx * ny * nz * c)), dim3(256), 0, (hipStream_t)stream,
+                           (const __bf16 *)in, (__bf16 *)out, nx, ny, nz, c, ctot, c0);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int ngapnet_up2_bwd(const float *dout, float *din, int nx, int ny, int nz, int c, int ctot, int c0, void *stream) {
-    hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for((long long)nx * ny * nz * c / 8)), dim3(256), 0,
-                       (hipStream_t)stream, dout, din, nx, ny, nz, c, ctot, c0);
+    if (c % 4 == 0 && ctot % 4 == 0 && c0 % 4 == 0 && ((uintptr_t)dout | (uintptr_t)din) % 16 == 0)
+        hipLaunchKernelGGL(k_up2_bwd_v4, dim3(blocks_for((long long)nx * ny * nz * c / 32)), dim3(256), 0,
+                           (hipStream_t)stream, dout, din, nx, ny, nz, c, ctot, c0);
+    else
+        hipLaunchKernelGGL(k_up2_bwd, dim3(blocks_for((long long)nx * ny * nz * c / 8)), dim3(256), 0,
+                           (hipStream_t)stream, dout, din, nx, ny, nz, c, ctot, c0);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int ngapnet_chan_copy_bf16(const void *src, void *dst, long long nv, int c, int cs, int s0, int cd, int d0,
                            void *stream) {
-    hipLaunchKernelGGL(k_chan_copy<__bf16>, dim3(blocks_for(nv * c)), dim3(256), 0, (hipStream_t)stream,
-                       (const __bf16 *)src, (__bf16 *)dst, nv, c, cs, s0, cd, d0);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    return chan_copy((const __bf16 *)src, (__bf16 *)dst, nv, c, cs, s0, cd, d0, stream);
 }
 
 int ngapnet_chan_copy_f32(const float *src, float *dst, long long nv, int c, int cs, int s0, int cd, int d0,
                           void *stream) {
-    hipLaunchKernelGGL(k_chan_copy<float>, dim3(blocks_for(nv * c)), dim3(256), 0, (hipStream_t)stream, src, dst, nv,
-                       c, cs, s0, cd, d0);
-    return hipGetLastError() == hipSuccess ? 0 : 1;
+    return chan_copy(src, dst, nv, c, cs, s0, cd, d0, stream);
 }
 
 int ngapnet_conv1_fwd(const void *x, const float *w, float *y, long long nv, int c, void *stream) {
@@ -711,7 +848,8 @@ int ngapnet_conv1_bwd(const void *x, const float *w, const float *dy, float *dx,
     else
         hipLaunchKernelGGL(k_conv1_bwd, dim3(nb), dim3(256), 0, (hipStream_t)stream, (const __bf16 *)x, w, dy, dx,
                            part, nv, c);
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, (hipStream_t)stream, (const float *)part, dw, nb, c);
+    hipLaunchKernelGGL(k_colsum, dim3((c + 15) / 16), dim3(256), 0, (hipStream_t)stream, (const float *)part, dw, nb,
+                       c);
     return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

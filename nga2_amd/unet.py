@@ -295,3 +295,44 @@ class UNet:
         self.set_weights({k: v for k, v in st.items() if k != "kappa"})
         if "kappa" in st:
             self.set_kappa(float(np.asarray(st["kappa"]).reshape(-1)[0]))
+
+
+def time_net(n=128, reps=5, device="cuda:0"):
+    """Milliseconds per forward pass and per training step (forward, backward,
+    Adam) of the net on an n^3 grid, random input and target, warm."""
+    import time
+
+    import torch
+
+    torch.zeros(1, device=device)
+    net = UNet((n, n, n), seed=0, device=device)
+    x = torch.randn(n ** 3, device=device)
+    t = torch.randn(n ** 3, device=device)
+    try:
+        for _ in range(3):
+            net.forward(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net.forward(x)
+        torch.cuda.synchronize()
+        fwd = (time.perf_counter() - t0) / reps * 1e3
+        for _ in range(2):
+            net.train_step(x, t, lr=1e-9)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            net.train_step(x, t, lr=1e-9)
+        torch.cuda.synchronize()
+        step = (time.perf_counter() - t0) / reps * 1e3
+    finally:
+        net.close()
+    return fwd, step
+
+
+if __name__ == "__main__":
+    import sys
+
+    _n = int(sys.argv[1]) if len(sys.argv) > 1 else 128
+    _fwd, _step = time_net(_n)
+    print(f"unet {_n}^3: forward {_fwd:.3f} ms, training step {_step:.3f} ms", flush=True)
