@@ -113,9 +113,23 @@ def build_fortran():
         exes.append(exe)
     return exes[0]
 
+UNS_SOURCES = ["ngap_amg.cpp", "ngap_uns.hip"]
+
+
+def build_uns_lib():
+    """libngapuns.so: hypre_uns (CSR + algebraic multigrid) on the GPU."""
+    os.makedirs(LIBDIR, exist_ok=True)
+    out = os.path.join(LIBDIR, "libngapuns.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in UNS_SOURCES]
+    _run([hipcc()] + HIPCC_FLAGS + ["-I" + INCLUDE, "-I" + CSRC] + srcs + ["-o", tmp])
+    os.replace(tmp, out)
+    return out
+
 
 def build_all(fortran=True):
-    out = {"solver": build_solver_lib(), "net": build_net_lib(), "fft": build_fft_lib(), "oracle": build_oracle()}
+    out = {"solver": build_solver_lib(), "net": build_net_lib(), "fft": build_fft_lib(), "uns": build_uns_lib(),
+           "oracle": build_oracle()}
     if fortran:
         out["fortran"] = build_fortran()
     return out

@@ -354,3 +354,53 @@ def implicit_problem(cfg, seed=4004, rho=1.0, visc=0.02, dt=None, cfl=1.0):
     rhs = cfg.field()
     rhs[cfg.interior] = rng.standard_normal((cfg.nx, cfg.ny, cfg.nz))
     return Problem(cfg, opr, rhs, f"implicit_{cfg.nx}x{cfg.ny}x{cfg.nz}", dict(seed=seed, rho=rho, visc=visc, dt=dt))
+
+
+def masked_spheres(n=128, seed=5005, nsph=40, rmin=0.04, rmax=0.12, no=1):
+    """Immersed solids as masked cells, the case hypre_uns serves: a periodic
+    unit cube n^3 with nsph solid spheres (centres and radii seeded as in
+    sphere_set); a cell whose centre lies inside a sphere (periodic images)
+    has VF = 0, so the masked metrics (incomp_class.f90:173-191, :516-566) give
+    it opr = 0 and its fluid neighbours a homogeneous Neumann face. The
+    right-hand side is N(0, 1) on the fluid cells with its mean removed over
+    each connected fluid region (the operator's null space), zero elsewhere."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    from . import incomp
+
+    cfg = Config(uniform(n), uniform(n), uniform(n), no=no)
+    centres, radii = sphere_set(nsph, rmin, rmax, seed)
+    h = 1.0 / n
+    xc = (np.arange(n) + 0.5) * h
+    solid = np.zeros((n, n, n), dtype=bool)
+    for c, r in zip(centres, radii):
+        dx = (xc - c[0] + 0.5) % 1.0 - 0.5
+        dy = (xc - c[1] + 0.5) % 1.0 - 0.5
+        dz = (xc - c[2] + 0.5) % 1.0 - 0.5
+        solid |= dx[:, None, None] ** 2 + dy[None, :, None] ** 2 + dz[None, None, :] ** 2 < r * r
+    it = cfg.interior
+    cfg.VF[it] = np.where(solid, 0.0, 1.0)
+    cfg.sync(cfg.VF)
+    opr = incomp.laplacian(incomp.Metrics(cfg))
+    rng = _rng(seed + 1)
+    rhs = cfg.field()
+    act = np.abs(opr[0][it]) > 10.0 * np.finfo(np.float64).eps
+    vals = np.where(act, rng.standard_normal((n, n, n)), 0.0)
+    # connected fluid regions through the nonzero couplings (periodic)
+    idx = np.arange(n ** 3).reshape((n, n, n), order="F")
+    rows, cols = [], []
+    for s, ax in ((1, 0), (3, 1), (5, 2)):
+        cpl = opr[s][it] != 0.0
+        rows.append(idx[cpl & act])
+        cols.append(np.roll(idx, -1, axis=ax)[cpl & act])
+    G = sp.coo_matrix((np.ones(sum(len(r) for r in rows)), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(n ** 3, n ** 3))
+    ncomp, lab = connected_components(G, directed=False)
+    lab = lab.reshape((n, n, n), order="F")
+    for c in np.unique(lab[act]):
+        m = act & (lab == c)
+        vals[m] -= vals[m].mean()
+    rhs[it] = vals
+    return Problem(cfg, opr, rhs, f"masked_spheres_{n}", dict(seed=seed, nsph=nsph, rmin=rmin, rmax=rmax,
+                                                              solid_cells=int(solid.sum()), regions=len(np.unique(lab[act]))))

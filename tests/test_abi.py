@@ -246,3 +246,17 @@ def test_convection_diffusion_operator():
         Coff = C - sp.diags(C.diagonal())
         assert abs(Coff + Coff.T).max() <= 1e-12 * abs(Coff).max()
         assert abs(Coff).max() > 0.0
+
+
+def test_uns_library_exports_all_declared_symbols():
+    from nga2_amd import uns
+
+    txt = open(os.path.join(ROOT, "include", "ngapuns.h")).read()
+    names = sorted(set(re.findall(r"^\s*(?:int|long long|void|const ngap_amg \*)\s*(ngap_(?:uns|amg)_\w+)\s*\(", txt,
+                                  re.M)))
+    assert len(names) >= 19
+    L = uns.lib()  # loads and sets every argtypes entry without touching the device
+    for n in names:
+        assert hasattr(L, n), n
+        assert getattr(L, n).argtypes is not None, n
+    assert set(uns.EXPORTS) == set(names)
