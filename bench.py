@@ -18,12 +18,16 @@ Outside the timed region the line also carries:
 - fourier3d, the FFT direct solve, at 256^3 triply periodic (nga2_amd.fourier),
   against the PCG solve of the same system;
 - gmres_pfmg on a nonsymmetric implicit-solve system on the 256^3 channel
-  grid, FP64 and with the FP32 V-cycle.
+  grid, FP64 and with the FP32 V-cycle;
+- hypre_uns's pcg_amg (algebraic multigrid over the CSR matrix of the
+  participating cells) on 128^3 immersed solids, against PCG+PFMG on the same
+  system.
 
   python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
                   [--no-cpu] [--cpu-threads T] [--roofline-reps R]
                   [--no-two-phase] [--no-sequence] [--seq-n 128] [--seq-m 8] [--seq-epochs 6]
                   [--no-drift] [--drift-n 256] [--drift-steps 24] [--no-fourier] [--no-implicit]
+                  [--no-uns] [--uns-n 128]
 """
 from __future__ import annotations
 
@@ -63,6 +67,8 @@ def parse():
     ap.add_argument("--no-drift", action="store_true", help="skip the drifting-spheres two-phase sequence leg")
     ap.add_argument("--no-fourier", action="store_true", help="skip the fourier3d (FFT direct solve) leg")
     ap.add_argument("--no-implicit", action="store_true", help="skip the gmres_pfmg implicit-solve leg")
+    ap.add_argument("--no-uns", action="store_true", help="skip the hypre_uns (pcg_amg) masked-solids leg")
+    ap.add_argument("--uns-n", type=int, default=128)
     ap.add_argument("--drift-n", type=int, default=256)
     ap.add_argument("--drift-steps", type=int, default=24)
     ap.add_argument("--seq-n", type=int, default=128)
@@ -284,6 +290,75 @@ def implicit_leg(args, dev, solves=3):
     return res
 
 
+def uns_leg(args, dev, solves=3):
+    """hypre_uns (pcg_amg) on immersed solids: the masked-spheres system at
+    uns_n^3 (40 solid spheres, Neumann at their faces, masked cells outside
+    the system): host hierarchy + upload, then the device solve from a zero
+    guess; the structured PCG+PFMG solve of the same system (masked cells as
+    identity rows) beside it."""
+    import torch
+
+    from nga2_amd import inputs, uns
+    from nga2_amd._lib import FP64, MIXED
+    from nga2_amd.linsol import DeviceSolve
+
+    t0 = time.time()
+    n = args.uns_n
+    P = inputs.masked_spheres(n=n)
+    cfg = P.cfg
+    log(f"[bench] generated {P.name} in {time.time() - t0:.1f}s ({P.info['solid_cells']} solid cells)")
+    d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
+    d_x = torch.zeros_like(d_rhs)
+    res = {"workload": f"{P.name}: {n}^3 periodic unit cube, {P.info['nsph']} solid spheres r "
+                       f"{P.info['rmin']}-{P.info['rmax']} (seed {P.info['seed']}) as masked cells with Neumann faces, "
+                       f"N(0,1) rhs on the fluid, rcvg {args.rcvg:g}, zero guess"}
+    t = time.perf_counter()
+    ds = uns.DeviceUns(cfg, P.opr, method=uns.PCG_AMG, maxit=200, rcvg=args.rcvg)
+    setup_ms = (time.perf_counter() - t) * 1e3
+    st = ds.stats()
+    ds.solve(d_rhs, d_x)  # warm-up
+    ms = []
+    for _ in range(solves):
+        d_x.zero_()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        it, rerr, _ = ds.solve(d_rhs, d_x)
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t) * 1e3)
+    st = ds.stats()
+    xa = d_x.clone()
+    res["pcg_amg"] = {"rows": st["rows"], "nnz": st["nnz"], "levels": st["nlevels"],
+                      "op_complexity": round(st["op_complexity"], 3), "setup_ms": round(setup_ms, 1),
+                      "hierarchy_host_ms": round(st["hier_ms"], 1), "iters": it, "true_rerr": rerr,
+                      "solve_ms": round(float(np.median(ms)), 3), "vcycle_ms": round(st["vcycle_ms"], 3)}
+    ds.destroy()
+    log(f"[bench] uns pcg_amg: rows {st['rows']}, levels {st['nlevels']}, it={it}, solve {res['pcg_amg']['solve_ms']} "
+        f"ms, V-cycle {st['vcycle_ms']:.3f} ms, setup {setup_ms:.0f} ms")
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    act = torch.from_numpy(np.ascontiguousarray((np.abs(P.opr[0]) > 1e-14).ravel(order="F"))).to(dev)
+    for name, prec in (("pcg_pfmg_fp64", FP64), ("pcg_pfmg_mixed", MIXED)):
+        s2 = DeviceSolve(cfg)
+        s2.setup(d_opr, maxit=200, rcvg=args.rcvg, precision=prec)
+        d_x.zero_()
+        s2.solve(d_rhs, d_x)
+        ms = []
+        for _ in range(solves):
+            d_x.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            it2, _ = s2.solve(d_rhs, d_x)
+            torch.cuda.synchronize()
+            ms.append((time.perf_counter() - t) * 1e3)
+        st2 = s2.stats()
+        a, b = xa[act], d_x[act]
+        diff = float(torch.linalg.vector_norm((a - a.mean()) - (b - b.mean())) / torch.linalg.vector_norm(b - b.mean()))
+        res[name] = {"iters": it2, "solve_ms": round(float(np.median(ms)), 3), "true_rerr": st2["true_rerr"],
+                     "rel_diff_vs_pcg_amg": diff}
+        s2.destroy()
+        log(f"[bench] uns same system, {name}: it={it2} solve {res[name]['solve_ms']} ms, diff {diff:.1e}")
+    return res
+
+
 def main():
     args = parse()
     ws = int(os.environ.get("WORLD_SIZE", "1"))
@@ -414,6 +489,10 @@ def main():
     if not args.no_implicit:
         impl_res = implicit_leg(args, dev)
 
+    uns_res = None
+    if not args.no_uns:
+        uns_res = uns_leg(args, dev)
+
     drift_res = None
     if not args.no_drift:
         from nga2_amd import drift
@@ -456,6 +535,8 @@ def main():
         out["fourier3d"] = fft_res
     if impl_res:
         out["implicit_gmres"] = impl_res
+    if uns_res:
+        out["uns_amg"] = uns_res
     print(json.dumps(out), flush=True)
     return 0
 

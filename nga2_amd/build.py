@@ -90,7 +90,7 @@ def flang():
 
 
 FORTRAN_DIR = os.path.join(PKG, "fortran")
-FORTRAN_SOURCES = ["ngap_c.f90", "ngap_str_class.f90", "ngap_fourier3d_class.f90"]
+FORTRAN_SOURCES = ["ngap_c.f90", "ngap_str_class.f90", "ngap_fourier3d_class.f90", "ngap_uns_class.f90"]
 FORTRAN_TEST_SOURCES = ["standalone/precision.f90", "standalone/config_class.f90", "standalone/linsol_class.f90"]
 
 
@@ -104,9 +104,10 @@ def build_fortran():
     bdir = os.path.join(LIBDIR, "fortran")
     os.makedirs(bdir, exist_ok=True)
     srcs = [os.path.join(FORTRAN_DIR, s) for s in FORTRAN_TEST_SOURCES + FORTRAN_SOURCES]
-    libs = ["-L" + LIBDIR, "-lngapsolve", "-lngapfft", "-Wl,-rpath," + LIBDIR]
+    libs = ["-L" + LIBDIR, "-lngapsolve", "-lngapfft", "-lngapuns", "-Wl,-rpath," + LIBDIR]
     exes = []
-    for drv, name in (("demo_driver.f90", "ngap_demo"), ("fft_demo.f90", "ngap_fft_demo")):
+    for drv, name in (("demo_driver.f90", "ngap_demo"), ("fft_demo.f90", "ngap_fft_demo"),
+                      ("uns_demo.f90", "ngap_uns_demo")):
         exe = os.path.join(bdir, name)
         _run([fc, "-O2", "-J", bdir] + srcs + [os.path.join(FORTRAN_DIR, drv)] + libs + ["-o", exe + ".tmp"], cwd=bdir)
         os.replace(exe + ".tmp", exe)

@@ -37,9 +37,13 @@ namespace {
         if (r_) return r_;        \
     } while (0)
 
-constexpr int NPART = 1024;  // partials of a dot product
+constexpr int NPART = 1024;     // partials of a dot product
+constexpr int MAX_VBLOCKS = 8192; // grid cap of the vector kernels (partials buffer size)
 
-enum { CSR_MUL = 0, CSR_RESID = 1, CSR_JACOBI = 2, CSR_ADD = 3 };
+// y = A x; y = f - A x; y = x + (f - A x) / d; y += A x; and, for the first
+// sweep from a zero guess (x = f / d, never stored): y = f - A (f / d) and
+// y = f / d + A x
+enum { CSR_MUL = 0, CSR_RESID = 1, CSR_JACOBI = 2, CSR_ADD = 3, CSR_RESID0 = 4, CSR_ADD0 = 5 };
 
 // ------------------------------------------------------------------ kernels
 template <int TPR, int MODE>
@@ -52,7 +56,14 @@ __global__ void __launch_bounds__(256) k_csr(int n, const int *__restrict__ rp, 
     if (row >= n) return;  // whole TPR groups leave together
     double s = 0.0;
     const int q1 = rp[row + 1];
-    for (int q = rp[row] + g; q < q1; q += TPR) s += val[q] * x[col[q]];
+    if (MODE == CSR_RESID0) {
+        for (int q = rp[row] + g; q < q1; q += TPR) {
+            const int c = col[q];
+            s += val[q] * __dmul_rn(f[c], dinv[c]);
+        }
+    } else {
+        for (int q = rp[row] + g; q < q1; q += TPR) s += val[q] * x[col[q]];
+    }
 #pragma unroll
     for (int off = TPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, TPR);
     if (g == 0) {
@@ -60,6 +71,8 @@ __global__ void __launch_bounds__(256) k_csr(int n, const int *__restrict__ rp, 
         if (MODE == CSR_RESID) y[row] = f[row] - s;
         if (MODE == CSR_JACOBI) y[row] = x[row] + (f[row] - s) * dinv[row];
         if (MODE == CSR_ADD) y[row] += s;
+        if (MODE == CSR_RESID0) y[row] = f[row] - s;
+        if (MODE == CSR_ADD0) y[row] = __dmul_rn(f[row], dinv[row]) + s;
     }
 }
 
@@ -102,6 +115,60 @@ __global__ void __launch_bounds__(256) k_fin(const double *__restrict__ part, in
     if (threadIdx.x == 0) out[slot] = (sm[0] + sm[1]) + (sm[2] + sm[3]);
 }
 
+// PCG scalars on the device: scal[0] rho = <r, z>, scal[1] <p, A p>,
+// scal[2] <r, r>, scal[3] alpha, scal[4] beta
+__device__ double block_sum(double s) {
+    __shared__ double sm[4];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    return (sm[0] + sm[1]) + (sm[2] + sm[3]);
+}
+
+// scal[1] = sum of the partials, alpha = rho / scal[1]
+__global__ void __launch_bounds__(256) k_fin_alpha(const double *__restrict__ part, int np, double *scal) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        scal[1] = s;
+        scal[3] = scal[0] / s;
+    }
+}
+
+// rho1 = sum of the partials, beta = rho1 / rho, rho = rho1
+__global__ void __launch_bounds__(256) k_fin_beta(const double *__restrict__ part, int np, double *scal) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) s += part[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        scal[4] = s / scal[0];
+        scal[0] = s;
+    }
+}
+
+// x += alpha p, r -= alpha q, one partial of <r, r> per workgroup
+__global__ void __launch_bounds__(256) k_pcg_xr(int n, const double *__restrict__ scal, const double *__restrict__ p,
+                                                const double *__restrict__ q, double *__restrict__ x,
+                                                double *__restrict__ r, double *__restrict__ part) {
+    const double a = scal[3];
+    double s = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        x[i] += a * p[i];
+        const double ri = r[i] - a * q[i];
+        r[i] = ri;
+        s += ri * ri;
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// p = z + beta p
+__global__ void k_pcg_p(int n, const double *__restrict__ scal, const double *__restrict__ z, double *__restrict__ p) {
+    const double b = scal[4];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) p[i] = z[i] + b * p[i];
+}
+
 // y = a x + b y (x may be y)
 __global__ void k_axpby(int n, double a, const double *x, double b, double *y) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) y[i] = a * x[i] + b * y[i];
@@ -131,7 +198,7 @@ __global__ void k_scatter(int n, const int *__restrict__ map, const double *__re
 
 int vec_blocks(long long n) {
     long long b = (n + 255) / 256;
-    if (b > 8192) b = 8192;
+    if (b > MAX_VBLOCKS) b = MAX_VBLOCKS;
     return (int)(b < 1 ? 1 : b);
 }
 
@@ -180,6 +247,12 @@ void csr_jacobi(hipStream_t s, const DCsr &M, const double *f, const double *din
 }
 void csr_add(hipStream_t s, const DCsr &M, const double *x, double *y) {
     launch_csr_mode<CSR_ADD>(s, M, x, nullptr, nullptr, y);
+}
+void csr_resid0(hipStream_t s, const DCsr &M, const double *f, const double *dinv, double *y) {
+    launch_csr_mode<CSR_RESID0>(s, M, nullptr, f, dinv, y);
+}
+void csr_add0(hipStream_t s, const DCsr &P, const double *f, const double *dinv, const double *x, double *y) {
+    launch_csr_mode<CSR_ADD0>(s, P, x, f, dinv, y);
 }
 
 }  // namespace
@@ -240,8 +313,11 @@ int upload_csr(ngap_uns *h, DCsr &D, const ngap::amg::Csr &M) {
     D.n = M.n;
     D.ncol = M.ncol;
     D.nnz = (long long)M.col.size();
+    // lanes per row: a quarter of the mean row length rounded up to a power of
+    // two (measured at 128^3 against a half, the whole and an eighth: 0.50 ms
+    // per V-cycle against 0.52, 0.63 and 0.70)
     const int mean = M.n > 0 ? (int)((D.nnz + M.n - 1) / M.n) : 1;
-    D.tpr = std::min(64, std::max(1, pow2ceil(mean)));
+    D.tpr = std::min(64, std::max(1, pow2ceil(mean) / 4));
     RCHK(upload(h, &D.rp, M.rp));
     RCHK(upload(h, &D.col, M.col));
     RCHK(upload(h, &D.val, M.val));
@@ -267,7 +343,9 @@ bool is_amg(int m) {
 // ------------------------------------------------------------------ cycle
 // One V(nu, nu) cycle from a zero guess, vout = M vin. On each level above the
 // coarsest the first sweep writes w and every later launch switches buffers,
-// so after 2 nu - 1 switches the result lies in x (vout on level 0).
+// so after 2 nu - 1 switches the result lies in x (vout on level 0). With one
+// sweep on a sparse level the first one (x = f / d) is never stored: the
+// residual reads f / d, and the prolongation writes f / d + P e into w.
 void vcycle(ngap_uns *h) {
     hipStream_t s = h->stream;
     const int nl = (int)h->L.size(), nu = h->o.nu;
@@ -282,14 +360,24 @@ void vcycle(ngap_uns *h) {
         const double *f = l == 0 ? h->vin : D.f;
         double *xf = l == 0 ? h->vout : D.x;
         double *bufs[2] = {D.w, xf};
-        int nb = 0;
-        hipLaunchKernelGGL(k_scale, dim3(vec_blocks(D.A.n)), dim3(256), 0, s, D.A.n, f, D.dinv, bufs[nb]);
-        for (int sw = 1; sw < nu; sw++) {
-            csr_jacobi(s, D.A, f, D.dinv, bufs[nb], bufs[nb ^ 1]);
-            nb ^= 1;
+        // fused only on sparse levels: with long rows the two gathers (f and
+        // 1/d) cost more than the stored x (measured at 128^3: 110 us fused
+        // against 86 us unfused at 30 nonzeros per row, 55 against 59 at 7)
+        if (nu == 1 && D.A.nnz <= 12LL * D.A.n) {
+            // the sweep from zero, x = f / d, stays implicit: the residual and
+            // the prolongation read f / d
+            cur[l] = nullptr;
+            csr_resid0(s, D.A, f, D.dinv, D.r);
+        } else {
+            int nb = 0;
+            hipLaunchKernelGGL(k_scale, dim3(vec_blocks(D.A.n)), dim3(256), 0, s, D.A.n, f, D.dinv, bufs[nb]);
+            for (int sw = 1; sw < nu; sw++) {
+                csr_jacobi(s, D.A, f, D.dinv, bufs[nb], bufs[nb ^ 1]);
+                nb ^= 1;
+            }
+            cur[l] = bufs[nb];
+            csr_resid(s, D.A, f, cur[l], D.r);
         }
-        cur[l] = bufs[nb];
-        csr_resid(s, D.A, f, cur[l], D.r);
         csr_mul(s, D.R, D.r, h->L[l + 1].f);
     }
     {
@@ -301,7 +389,12 @@ void vcycle(ngap_uns *h) {
         const double *f = l == 0 ? h->vin : D.f;
         double *xf = l == 0 ? h->vout : D.x;
         double *c = cur[l];
-        csr_add(s, D.P, h->L[l + 1].x, c);
+        if (!c) {
+            c = D.w;
+            csr_add0(s, D.P, f, D.dinv, h->L[l + 1].x, c);
+        } else {
+            csr_add(s, D.P, h->L[l + 1].x, c);
+        }
         for (int sw = 0; sw < nu; sw++) {
             double *nxt = c == xf ? D.w : xf;
             csr_jacobi(s, D.A, f, D.dinv, c, nxt);
@@ -363,10 +456,13 @@ int residual(ngap_uns *h, double *nrm) {
 }
 
 // ------------------------------------------------------------------ Krylov drivers
-// r is kept in vin, so PCG's z = M r needs no copy.
+// r is kept in vin, so PCG's z = M r needs no copy. alpha, beta and rho stay
+// on the device; the host reads <p, Ap> and <r, r> once per iteration for the
+// breakdown and stopping tests.
 int run_pcg(ngap_uns *h, double bn, int &it) {
     hipStream_t s = h->stream;
     const DCsr &A = h->L[0].A;
+    const int n = h->rows, nb = vec_blocks(n);
     double *r = h->vin, *z = h->vout;
     csr_resid(s, A, h->b, h->x, r);
     double rr;
@@ -374,24 +470,25 @@ int run_pcg(ngap_uns *h, double bn, int &it) {
     if (std::sqrt(rr) / bn < h->rcvg) return 0;
     RCHK(precond(h));
     copy(h, z, h->p);
-    double rho;
-    RCHK(dot(h, r, z, &rho));
+    hipLaunchKernelGGL(k_dot, dim3(NPART), dim3(256), 0, s, n, r, z, h->part);
+    hipLaunchKernelGGL(k_fin, dim3(1), dim3(256), 0, s, h->part, NPART, h->scal, 0);
     while (it < h->maxit) {
         csr_mul(s, A, h->p, h->q);
-        double pq;
-        RCHK(dot(h, h->p, h->q, &pq));
+        hipLaunchKernelGGL(k_dot, dim3(NPART), dim3(256), 0, s, n, h->p, h->q, h->part);
+        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(256), 0, s, h->part, NPART, h->scal);
+        hipLaunchKernelGGL(k_pcg_xr, dim3(nb), dim3(256), 0, s, n, h->scal, h->p, h->q, h->x, r, h->part);
+        hipLaunchKernelGGL(k_fin, dim3(1), dim3(256), 0, s, h->part, nb, h->scal, 2);
+        UCHK(hipMemcpyAsync(h->h_scal, h->scal, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
+        UCHK(hipStreamSynchronize(s));
+        const double pq = h->h_scal[1];
+        rr = h->h_scal[2];
         if (!(pq > 0.0) || !std::isfinite(pq)) return NGAP_ERR_BREAKDOWN;
-        const double alpha = rho / pq;
-        axpby(h, alpha, h->p, 1.0, h->x);
-        axpby(h, -alpha, h->q, 1.0, r);
         it++;
-        RCHK(dot(h, r, r, &rr));
         if (std::sqrt(rr) / bn < h->rcvg) break;
         RCHK(precond(h));
-        double rho1;
-        RCHK(dot(h, r, z, &rho1));
-        axpby(h, 1.0, z, rho1 / rho, h->p);
-        rho = rho1;
+        hipLaunchKernelGGL(k_dot, dim3(NPART), dim3(256), 0, s, n, r, z, h->part);
+        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, s, h->part, NPART, h->scal);
+        hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(256), 0, s, n, h->scal, z, h->p);
     }
     return 0;
 }
@@ -730,7 +827,7 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     }
     RCHK(dalloc(h, (void **)&h->grhs, sizeof(double) * h->ntot));
     RCHK(dalloc(h, (void **)&h->gsol, sizeof(double) * h->ntot));
-    RCHK(dalloc(h, (void **)&h->part, sizeof(double) * NPART));
+    RCHK(dalloc(h, (void **)&h->part, sizeof(double) * MAX_VBLOCKS));
     RCHK(dalloc(h, (void **)&h->scal, sizeof(double) * 8));
     UCHK(hipHostMalloc((void **)&h->h_scal, sizeof(double) * 8));
     // the V-cycle as a graph on the handle's stream

@@ -46,6 +46,14 @@ module ngap_c
    public :: ngap_destroy,ngap_last_stats,ngap_strerror_f
    ! fourier3d on the GPU (libngapfft.so, include/ngapfft.h)
    public :: ngap_fft_create,ngap_fft_setup,ngap_fft_solve,ngap_fft_destroy
+   ! hypre_uns on the GPU (libngapuns.so, include/ngapuns.h)
+   public :: ngap_uns_default_opts,ngap_uns_create,ngap_uns_init,ngap_uns_setup,ngap_uns_solve,ngap_uns_destroy
+   public :: ngap_uns_rows,ngap_uns_index
+
+   type, bind(C), public :: ngap_uns_opts_t
+      real(c_double) :: strong_threshold,max_row_sum
+      integer(c_int) :: pmax,max_coarse,max_levels,nu,kdim,graphs,verbose
+   end type ngap_uns_opts_t
 
    interface
       integer(c_int) function ngap_default_opts(o) bind(C,name='ngap_default_opts')
@@ -128,6 +136,52 @@ module ngap_c
          import :: c_int,c_ptr
          type(c_ptr), value :: f
       end function ngap_fft_destroy
+      integer(c_int) function ngap_uns_default_opts(o) bind(C,name='ngap_uns_default_opts')
+         import :: c_int,ngap_uns_opts_t
+         type(ngap_uns_opts_t), intent(out) :: o
+      end function ngap_uns_default_opts
+      integer(c_int) function ngap_uns_create(h,g,nst,stc,method) bind(C,name='ngap_uns_create')
+         import :: c_int,c_ptr,ngap_grid_t
+         type(c_ptr), intent(out) :: h
+         type(ngap_grid_t), intent(in) :: g
+         integer(c_int), value :: nst
+         integer(c_int), intent(in) :: stc(*)
+         integer(c_int), value :: method
+      end function ngap_uns_create
+      integer(c_int) function ngap_uns_init(h,opr) bind(C,name='ngap_uns_init')
+         import :: c_int,c_ptr,c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: opr(*)
+      end function ngap_uns_init
+      integer(c_int) function ngap_uns_setup(h,opr,maxit,rcvg,o) bind(C,name='ngap_uns_setup')
+         import :: c_int,c_ptr,c_double,ngap_uns_opts_t
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: opr(*)
+         integer(c_int), value :: maxit
+         real(c_double), value :: rcvg
+         type(ngap_uns_opts_t), intent(in) :: o
+      end function ngap_uns_setup
+      integer(c_int) function ngap_uns_solve(h,rhs,sol,it,rerr,aerr) bind(C,name='ngap_uns_solve')
+         import :: c_int,c_ptr,c_double
+         type(c_ptr), value :: h
+         real(c_double), intent(in) :: rhs(*)
+         real(c_double), intent(inout) :: sol(*)
+         integer(c_int), intent(out) :: it
+         real(c_double), intent(out) :: rerr,aerr
+      end function ngap_uns_solve
+      integer(c_int) function ngap_uns_destroy(h) bind(C,name='ngap_uns_destroy')
+         import :: c_int,c_ptr
+         type(c_ptr), value :: h
+      end function ngap_uns_destroy
+      integer(c_long_long) function ngap_uns_rows(h) bind(C,name='ngap_uns_rows')
+         import :: c_long_long,c_ptr
+         type(c_ptr), value :: h
+      end function ngap_uns_rows
+      integer(c_int) function ngap_uns_index(h,ind) bind(C,name='ngap_uns_index')
+         import :: c_int,c_ptr
+         type(c_ptr), value :: h
+         integer(c_int), intent(out) :: ind(*)
+      end function ngap_uns_index
       type(c_ptr) function ngap_strerror(code) bind(C,name='ngap_strerror')
          import :: c_int,c_ptr
          integer(c_int), value :: code
