@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 
 namespace ngap {
 namespace amg {
@@ -28,6 +29,29 @@ namespace amg {
 namespace {
 
 constexpr int C_PT = 1, F_PT = -1, U_PT = 0;
+
+// Row ranges on host threads (at most 16). Every row is computed by the same
+// code whichever thread runs it, and per-thread outputs are joined in row
+// order, so the hierarchy does not depend on the thread count.
+int nthreads_for(long long n) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const long long cap = std::min<long long>(16, hw ? (long long)hw : 1);
+    return (int)std::max<long long>(1, std::min<long long>(cap, n / 8192));
+}
+
+template <typename F>
+void par_rows(int n, int nt, F &&body) {
+    if (nt <= 1) {
+        body(0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) {
+        const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
+        th.emplace_back([&body, lo, hi, t] { body(lo, hi, t); });
+    }
+    for (auto &x : th) x.join();
+}
 
 double rand01(long long i) {
     uint64_t z = (uint64_t)i + 0x9E3779B97F4A7C15ULL;
@@ -47,7 +71,8 @@ double diag_of(const Csr &A, int i) {
 // S[q]: row i strongly depends on A.col[q]
 std::vector<char> strength(const Csr &A, double theta, double max_row_sum) {
     std::vector<char> S(A.col.size(), 0);
-    for (int i = 0; i < A.n; i++) {
+    par_rows(A.n, nthreads_for(A.n), [&](int lo, int hi, int) {
+    for (int i = lo; i < hi; i++) {
         const int a = A.rp[i], b = A.rp[i + 1];
         const double diag = diag_of(A, i);
         double rsum = 0.0;
@@ -77,6 +102,7 @@ std::vector<char> strength(const Csr &A, double theta, double max_row_sum) {
                 if (A.col[q] != i && A.val[q] > thr) S[q] = 1;
         }
     }
+    });
     return S;
 }
 
@@ -126,31 +152,43 @@ std::vector<int> pmis(const Csr &A, const Graph &G) {
         for (int i = 0; i < n; i++)
             if (cf[i] == U_PT) und.push_back(i);
         if (und.empty()) break;
+        const int nu = (int)und.size(), nt = nthreads_for(nu);
+        // new C points: undecided points beating every undecided strong neighbour
+        // (reads the round's starting state only)
+        std::vector<std::vector<int>> part(nt);
+        par_rows(nu, nt, [&](int lo, int hi, int t) {
+            for (int u = lo; u < hi; u++) {
+                const int i = und[u];
+                bool best = true;
+                for (int q = G.drp[i]; q < G.drp[i + 1] && best; q++) {
+                    const int j = G.dep[q];
+                    if (cf[j] == U_PT && !(w[i] > w[j])) best = false;
+                }
+                for (int q = G.irp[i]; q < G.irp[i + 1] && best; q++) {
+                    const int j = G.inf[q];
+                    if (cf[j] == U_PT && !(w[i] > w[j])) best = false;
+                }
+                if (best) part[t].push_back(i);
+            }
+        });
         newc.clear();
-        for (int i : und) {
-            bool best = true;
-            for (int q = G.drp[i]; q < G.drp[i + 1] && best; q++) {
-                const int j = G.dep[q];
-                if (cf[j] == U_PT && !(w[i] > w[j])) best = false;
-            }
-            for (int q = G.irp[i]; q < G.irp[i + 1] && best; q++) {
-                const int j = G.inf[q];
-                if (cf[j] == U_PT && !(w[i] > w[j])) best = false;
-            }
-            if (best) newc.push_back(i);
-        }
+        for (auto &pt : part) newc.insert(newc.end(), pt.begin(), pt.end());
         for (int i : newc) {
             cf[i] = C_PT;
             isnew[i] = 1;
         }
-        for (int i : und) {
-            if (cf[i] != U_PT) continue;
-            for (int q = G.drp[i]; q < G.drp[i + 1]; q++)
-                if (isnew[G.dep[q]]) {
-                    cf[i] = F_PT;
-                    break;
-                }
-        }
+        // undecided points depending strongly on a new C point become F
+        par_rows(nu, nt, [&](int lo, int hi, int) {
+            for (int u = lo; u < hi; u++) {
+                const int i = und[u];
+                if (cf[i] != U_PT) continue;
+                for (int q = G.drp[i]; q < G.drp[i + 1]; q++)
+                    if (isnew[G.dep[q]]) {
+                        cf[i] = F_PT;
+                        break;
+                    }
+            }
+        });
         for (int i : newc) isnew[i] = 0;
     }
     return cf;
@@ -201,27 +239,21 @@ void truncate_row(std::vector<int> &cols, std::vector<double> &vals, int pmax) {
     vals.swap(v3);
 }
 
-Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int pmax) {
+// rows [lo, hi) of P appended to (col, val), row lengths to len
+void interp_rows(const Csr &A, const Graph &G, const std::vector<int> &cf, const std::vector<int> &cmap,
+                 const std::vector<double> &adiag, int pmax, int lo, int hi, std::vector<int> &pcol,
+                 std::vector<double> &pval, std::vector<int> &len) {
     const int n = A.n;
-    std::vector<int> cmap(n, -1);
-    int nc = 0;
-    for (int i = 0; i < n; i++)
-        if (cf[i] == C_PT) cmap[i] = nc++;
-    std::vector<double> adiag(n);
-    for (int i = 0; i < n; i++) adiag[i] = diag_of(A, i);
-    Csr P;
-    P.n = n;
-    P.ncol = nc;
-    P.rp.assign(n + 1, 0);
     // marks: in C-hat (position + 1), strong dependency of i (stamp)
     std::vector<int> chat_pos(n, 0), sstamp(n, -1);
     std::vector<int> chat, cols;
     std::vector<double> wv, vals;
-    for (int i = 0; i < n; i++) {
+    for (int i = lo; i < hi; i++) {
+        const size_t before = pcol.size();
         if (cf[i] == C_PT) {
-            P.col.push_back(cmap[i]);
-            P.val.push_back(1.0);
-            P.rp[i + 1] = (int)P.col.size();
+            pcol.push_back(cmap[i]);
+            pval.push_back(1.0);
+            len.push_back(1);
             continue;
         }
         chat.clear();
@@ -288,11 +320,40 @@ Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int p
                 vals[q] = -wv[q] / diag;
             }
             truncate_row(cols, vals, pmax);
-            P.col.insert(P.col.end(), cols.begin(), cols.end());
-            P.val.insert(P.val.end(), vals.begin(), vals.end());
+            pcol.insert(pcol.end(), cols.begin(), cols.end());
+            pval.insert(pval.end(), vals.begin(), vals.end());
         }
         for (int j : chat) chat_pos[j] = 0;
-        P.rp[i + 1] = (int)P.col.size();
+        len.push_back((int)(pcol.size() - before));
+    }
+}
+
+Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int pmax) {
+    const int n = A.n;
+    std::vector<int> cmap(n, -1);
+    int nc = 0;
+    for (int i = 0; i < n; i++)
+        if (cf[i] == C_PT) cmap[i] = nc++;
+    std::vector<double> adiag(n);
+    for (int i = 0; i < n; i++) adiag[i] = diag_of(A, i);
+    const int nt = nthreads_for(n);
+    std::vector<std::vector<int>> pcol(nt), len(nt);
+    std::vector<std::vector<double>> pval(nt);
+    par_rows(n, nt, [&](int lo, int hi, int t) {
+        interp_rows(A, G, cf, cmap, adiag, pmax, lo, hi, pcol[t], pval[t], len[t]);
+    });
+    Csr P;
+    P.n = n;
+    P.ncol = nc;
+    P.rp.assign(n + 1, 0);
+    int row = 0;
+    for (int t = 0; t < nt; t++) {
+        for (int l : len[t]) {
+            P.rp[row + 1] = P.rp[row] + l;
+            row++;
+        }
+        P.col.insert(P.col.end(), pcol[t].begin(), pcol[t].end());
+        P.val.insert(P.val.end(), pval[t].begin(), pval[t].end());
     }
     return P;
 }
@@ -318,36 +379,111 @@ Csr transpose(const Csr &A) {
     return T;
 }
 
-// C = A B, row by row with a dense accumulator; columns ascending
+// C = A B, columns of each row ascending. Two passes over the rows: the
+// first counts each row's distinct columns, the second writes columns and
+// sums in place. A row with up to HCAP/2 products accumulates in a small
+// open-addressing table that stays in cache, a longer one in a dense array.
+// Each column sums its products in the order of A's row, then B's row.
+namespace {
+constexpr int HCAP = 4096;
+
+struct RowAcc {
+    std::vector<int> hkey, cols, slots, mark;
+    std::vector<double> hval, acc;
+    RowAcc() : hkey(HCAP, -1), hval(HCAP) {}
+
+    // the distinct columns of row i of A B (ascending) in cols; with sums when
+    // vals is set (read back through column())
+    void row(const Csr &A, const Csr &B, int i, bool vals) {
+        cols.clear();
+        long long bound = 0;
+        for (int q = A.rp[i]; q < A.rp[i + 1]; q++) bound += B.rp[A.col[q] + 1] - B.rp[A.col[q]];
+        dense = bound > HCAP / 2;
+        if (!dense) {
+            slots.clear();
+            for (int q = A.rp[i]; q < A.rp[i + 1]; q++) {
+                const int k = A.col[q];
+                const double a = A.val[q];
+                for (int u = B.rp[k]; u < B.rp[k + 1]; u++) {
+                    const int j = B.col[u];
+                    unsigned h = ((unsigned)j * 2654435761u) & (HCAP - 1);
+                    while (hkey[h] != -1 && hkey[h] != j) h = (h + 1) & (HCAP - 1);
+                    if (hkey[h] == -1) {
+                        hkey[h] = j;
+                        hval[h] = 0.0;
+                        slots.push_back((int)h);
+                        cols.push_back(j);
+                    }
+                    if (vals) hval[h] += a * B.val[u];
+                }
+            }
+        } else {
+            if (acc.empty()) {
+                acc.assign(B.ncol, 0.0);
+                mark.assign(B.ncol, -1);
+            }
+            for (int q = A.rp[i]; q < A.rp[i + 1]; q++) {
+                const int k = A.col[q];
+                const double a = A.val[q];
+                for (int u = B.rp[k]; u < B.rp[k + 1]; u++) {
+                    const int j = B.col[u];
+                    if (mark[j] != i) {
+                        mark[j] = i;
+                        acc[j] = 0.0;
+                        cols.push_back(j);
+                    }
+                    if (vals) acc[j] += a * B.val[u];
+                }
+            }
+            // a later row reuses mark: clear this row's entries
+            for (int j : cols) mark[j] = -1;
+        }
+        std::sort(cols.begin(), cols.end());
+    }
+    double column(int j) const {
+        if (dense) return acc[j];
+        unsigned h = ((unsigned)j * 2654435761u) & (HCAP - 1);
+        while (hkey[h] != j) h = (h + 1) & (HCAP - 1);
+        return hval[h];
+    }
+    void clear() {
+        if (!dense)
+            for (int h : slots) hkey[h] = -1;
+    }
+    bool dense = false;
+};
+}  // namespace
+
 Csr spgemm(const Csr &A, const Csr &B) {
+    const int nt = nthreads_for(A.n);
     Csr C;
     C.n = A.n;
     C.ncol = B.ncol;
     C.rp.assign(A.n + 1, 0);
-    std::vector<double> acc(B.ncol, 0.0);
-    std::vector<int> mark(B.ncol, -1), cols;
-    for (int i = 0; i < A.n; i++) {
-        cols.clear();
-        for (int q = A.rp[i]; q < A.rp[i + 1]; q++) {
-            const int k = A.col[q];
-            const double a = A.val[q];
-            for (int t = B.rp[k]; t < B.rp[k + 1]; t++) {
-                const int j = B.col[t];
-                if (mark[j] != i) {
-                    mark[j] = i;
-                    acc[j] = 0.0;
-                    cols.push_back(j);
-                }
-                acc[j] += a * B.val[t];
+    par_rows(A.n, nt, [&](int lo, int hi, int) {
+        RowAcc r;
+        for (int i = lo; i < hi; i++) {
+            r.row(A, B, i, false);
+            C.rp[i + 1] = (int)r.cols.size();
+            r.clear();
+        }
+    });
+    for (int i = 0; i < A.n; i++) C.rp[i + 1] += C.rp[i];
+    C.col.resize(C.rp[A.n]);
+    C.val.resize(C.rp[A.n]);
+    par_rows(A.n, nt, [&](int lo, int hi, int) {
+        RowAcc r;
+        for (int i = lo; i < hi; i++) {
+            r.row(A, B, i, true);
+            int p = C.rp[i];
+            for (int j : r.cols) {
+                C.col[p] = j;
+                C.val[p] = r.column(j);
+                p++;
             }
+            r.clear();
         }
-        std::sort(cols.begin(), cols.end());
-        for (int j : cols) {
-            C.col.push_back(j);
-            C.val.push_back(acc[j]);
-        }
-        C.rp[i + 1] = (int)C.col.size();
-    }
+    });
     return C;
 }
 
@@ -426,11 +562,13 @@ int build(Hier &H, Csr A, const ngap_uns_opts &o) {
     H.d.clear();
     for (const Csr &L : H.A) {
         std::vector<double> d(L.n);
-        for (int i = 0; i < L.n; i++) {
-            double s = 0.0;
-            for (int q = L.rp[i]; q < L.rp[i + 1]; q++) s += std::fabs(L.val[q]);
-            d[i] = s;
-        }
+        par_rows(L.n, nthreads_for(L.n), [&](int lo, int hi, int) {
+            for (int i = lo; i < hi; i++) {
+                double s = 0.0;
+                for (int q = L.rp[i]; q < L.rp[i + 1]; q++) s += std::fabs(L.val[q]);
+                d[i] = s;
+            }
+        });
         H.d.push_back(std::move(d));
     }
     return coarse_inverse(H.A.back(), H.Ainv, H.singular);
