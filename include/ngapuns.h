@@ -49,6 +49,9 @@ typedef struct ngap_uns_opts {
     int kdim;                /* GMRES restart length, 5 (gmres_kdim, :28) */
     int graphs;              /* replay the V-cycle as a HIP graph, 1 */
     int verbose;             /* 1: one line per setup and solve on stderr */
+    int precision;           /* NGAP_FP64 (default), or NGAP_MIXED: the multigrid cycle in FP32 under the
+                                FP64 Krylov iteration (PCG then takes the flexible beta); no counterpart
+                                in hypre_uns */
 } ngap_uns_opts;
 
 typedef struct ngap_uns_stats {

@@ -673,6 +673,7 @@ int ngap_uns_default_opts(ngap_uns_opts *o) {
     o->kdim = 5;
     o->graphs = 1;
     o->verbose = 0;
+    o->precision = NGAP_FP64;
     return 0;
 }
 

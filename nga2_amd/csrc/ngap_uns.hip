@@ -46,20 +46,24 @@ constexpr int MAX_VBLOCKS = 8192; // grid cap of the vector kernels (partials bu
 enum { CSR_MUL = 0, CSR_RESID = 1, CSR_JACOBI = 2, CSR_ADD = 3, CSR_RESID0 = 4, CSR_ADD0 = 5 };
 
 // ------------------------------------------------------------------ kernels
-template <int TPR, int MODE>
+__device__ __forceinline__ double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ __forceinline__ float mul_rn(float a, float b) { return __fmul_rn(a, b); }
+
+// T: the cycle's value type (FP64, or FP32 for the mixed-precision cycle)
+template <typename T, int TPR, int MODE>
 __global__ void __launch_bounds__(256) k_csr(int n, const int *__restrict__ rp, const int *__restrict__ col,
-                                             const double *__restrict__ val, const double *__restrict__ x,
-                                             const double *__restrict__ f, const double *__restrict__ dinv,
-                                             double *__restrict__ y) {
+                                             const T *__restrict__ val, const T *__restrict__ x,
+                                             const T *__restrict__ f, const T *__restrict__ dinv,
+                                             T *__restrict__ y) {
     const long long t = blockIdx.x * 256LL + threadIdx.x;
     const int row = (int)(t / TPR), g = (int)(t % TPR);
     if (row >= n) return;  // whole TPR groups leave together
-    double s = 0.0;
+    T s = T(0);
     const int q1 = rp[row + 1];
     if (MODE == CSR_RESID0) {
         for (int q = rp[row] + g; q < q1; q += TPR) {
             const int c = col[q];
-            s += val[q] * __dmul_rn(f[c], dinv[c]);
+            s += val[q] * mul_rn(f[c], dinv[c]);
         }
     } else {
         for (int q = rp[row] + g; q < q1; q += TPR) s += val[q] * x[col[q]];
@@ -72,23 +76,33 @@ __global__ void __launch_bounds__(256) k_csr(int n, const int *__restrict__ rp, 
         if (MODE == CSR_JACOBI) y[row] = x[row] + (f[row] - s) * dinv[row];
         if (MODE == CSR_ADD) y[row] += s;
         if (MODE == CSR_RESID0) y[row] = f[row] - s;
-        if (MODE == CSR_ADD0) y[row] = __dmul_rn(f[row], dinv[row]) + s;
+        if (MODE == CSR_ADD0) y[row] = mul_rn(f[row], dinv[row]) + s;
     }
 }
 
-__global__ void k_scale(int n, const double *__restrict__ f, const double *__restrict__ dinv, double *__restrict__ x) {
+template <typename T>
+__global__ void k_scale(int n, const T *__restrict__ f, const T *__restrict__ dinv, T *__restrict__ x) {
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[i] = f[i] * dinv[i];
 }
 
-// coarsest level: x = Ainv f, one wavefront per row
-__global__ void __launch_bounds__(256) k_dense(int n, const double *__restrict__ Ainv, const double *__restrict__ f,
-                                               double *__restrict__ x) {
+// coarsest level: x = Ainv f, one wavefront per row, FP64 sums
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense(int n, const double *__restrict__ Ainv, const T *__restrict__ f,
+                                               T *__restrict__ x) {
     const int row = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (row >= n) return;
     double s = 0.0;
-    for (int j = lane; j < n; j += 64) s += Ainv[(long long)row * n + j] * f[j];
+    for (int j = lane; j < n; j += 64) s += Ainv[(long long)row * n + j] * (double)f[j];
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    if (lane == 0) x[row] = s;
+    if (lane == 0) x[row] = (T)s;
+}
+
+// precision conversions around the FP32 cycle
+__global__ void k_to_f32(int n, const double *__restrict__ a, float *__restrict__ b) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (float)a[i];
+}
+__global__ void k_to_f64(int n, const float *__restrict__ a, double *__restrict__ b) {
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (double)a[i];
 }
 
 // one FP64 partial of <a, b> per workgroup (grid-stride, NPART workgroups)
@@ -163,6 +177,39 @@ __global__ void __launch_bounds__(256) k_pcg_xr(int n, const double *__restrict_
     if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// partials of <r, z> and <z, q> (flexible PCG), two per workgroup
+__global__ void __launch_bounds__(256) k_dot2(int n, const double *__restrict__ r, const double *__restrict__ z,
+                                              const double *__restrict__ q, double *__restrict__ part) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        s1 += r[i] * z[i];
+        s2 += z[i] * q[i];
+    }
+    s1 = block_sum(s1);
+    __syncthreads();
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s1;
+        part[2 * blockIdx.x + 1] = s2;
+    }
+}
+
+// flexible beta = -alpha <z1, q> / rho (r1 - r = -alpha q), rho = <r1, z1>
+__global__ void __launch_bounds__(256) k_fin_beta_flex(const double *__restrict__ part, int np, double *scal) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < np; i += 256) {
+        s1 += part[2 * i];
+        s2 += part[2 * i + 1];
+    }
+    s1 = block_sum(s1);
+    __syncthreads();
+    s2 = block_sum(s2);
+    if (threadIdx.x == 0) {
+        scal[4] = -scal[3] * s2 / scal[0];
+        scal[0] = s1;
+    }
+}
+
 // p = z + beta p
 __global__ void k_pcg_p(int n, const double *__restrict__ scal, const double *__restrict__ z, double *__restrict__ p) {
     const double b = scal[4];
@@ -214,45 +261,63 @@ struct DCsr {
     long long nnz = 0;
     int *rp = nullptr, *col = nullptr;
     double *val = nullptr;
+    float *val32 = nullptr;  // FP32 copy for the mixed-precision cycle
+    const double *v(double) const { return val; }
+    const float *v(float) const { return val32; }
+};
+
+template <typename T>
+struct LVec {
+    T *dinv = nullptr, *f = nullptr, *x = nullptr, *w = nullptr, *r = nullptr;
 };
 
 struct DLevel {
     DCsr A, P, R;  // P: this level's rows x next level's rows; R = P^T
-    double *dinv = nullptr, *f = nullptr, *x = nullptr, *w = nullptr, *r = nullptr;
+    LVec<double> d;
+    LVec<float> s;
+    LVec<double> &vec(double) { return d; }
+    LVec<float> &vec(float) { return s; }
 };
 
-template <int MODE>
-void launch_csr_mode(hipStream_t s, const DCsr &M, const double *x, const double *f, const double *dinv, double *y) {
+template <typename T, int MODE>
+void launch_csr_mode(hipStream_t s, const DCsr &M, const T *x, const T *f, const T *dinv, T *y) {
     const long long threads = (long long)M.n * M.tpr;
     const dim3 grid((unsigned)((threads + 255) / 256));
+    const T *val = M.v(T());
     switch (M.tpr) {
-        case 1: hipLaunchKernelGGL((k_csr<1, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
-        case 2: hipLaunchKernelGGL((k_csr<2, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
-        case 4: hipLaunchKernelGGL((k_csr<4, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
-        case 8: hipLaunchKernelGGL((k_csr<8, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
-        case 16: hipLaunchKernelGGL((k_csr<16, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
-        case 32: hipLaunchKernelGGL((k_csr<32, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
-        default: hipLaunchKernelGGL((k_csr<64, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, M.val, x, f, dinv, y); break;
+        case 1: hipLaunchKernelGGL((k_csr<T, 1, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
+        case 2: hipLaunchKernelGGL((k_csr<T, 2, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
+        case 4: hipLaunchKernelGGL((k_csr<T, 4, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
+        case 8: hipLaunchKernelGGL((k_csr<T, 8, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
+        case 16: hipLaunchKernelGGL((k_csr<T, 16, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
+        case 32: hipLaunchKernelGGL((k_csr<T, 32, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
+        default: hipLaunchKernelGGL((k_csr<T, 64, MODE>), grid, dim3(256), 0, s, M.n, M.rp, M.col, val, x, f, dinv, y); break;
     }
 }
 
-void csr_mul(hipStream_t s, const DCsr &M, const double *x, double *y) {
-    launch_csr_mode<CSR_MUL>(s, M, x, nullptr, nullptr, y);
+template <typename T>
+void csr_mul(hipStream_t s, const DCsr &M, const T *x, T *y) {
+    launch_csr_mode<T, CSR_MUL>(s, M, x, nullptr, nullptr, y);
 }
-void csr_resid(hipStream_t s, const DCsr &M, const double *f, const double *x, double *y) {
-    launch_csr_mode<CSR_RESID>(s, M, x, f, nullptr, y);
+template <typename T>
+void csr_resid(hipStream_t s, const DCsr &M, const T *f, const T *x, T *y) {
+    launch_csr_mode<T, CSR_RESID>(s, M, x, f, nullptr, y);
 }
-void csr_jacobi(hipStream_t s, const DCsr &M, const double *f, const double *dinv, const double *x, double *y) {
-    launch_csr_mode<CSR_JACOBI>(s, M, x, f, dinv, y);
+template <typename T>
+void csr_jacobi(hipStream_t s, const DCsr &M, const T *f, const T *dinv, const T *x, T *y) {
+    launch_csr_mode<T, CSR_JACOBI>(s, M, x, f, dinv, y);
 }
-void csr_add(hipStream_t s, const DCsr &M, const double *x, double *y) {
-    launch_csr_mode<CSR_ADD>(s, M, x, nullptr, nullptr, y);
+template <typename T>
+void csr_add(hipStream_t s, const DCsr &M, const T *x, T *y) {
+    launch_csr_mode<T, CSR_ADD>(s, M, x, nullptr, nullptr, y);
 }
-void csr_resid0(hipStream_t s, const DCsr &M, const double *f, const double *dinv, double *y) {
-    launch_csr_mode<CSR_RESID0>(s, M, nullptr, f, dinv, y);
+template <typename T>
+void csr_resid0(hipStream_t s, const DCsr &M, const T *f, const T *dinv, T *y) {
+    launch_csr_mode<T, CSR_RESID0>(s, M, nullptr, f, dinv, y);
 }
-void csr_add0(hipStream_t s, const DCsr &P, const double *f, const double *dinv, const double *x, double *y) {
-    launch_csr_mode<CSR_ADD0>(s, P, x, f, dinv, y);
+template <typename T>
+void csr_add0(hipStream_t s, const DCsr &P, const T *f, const T *dinv, const T *x, T *y) {
+    launch_csr_mode<T, CSR_ADD0>(s, P, x, f, dinv, y);
 }
 
 }  // namespace
@@ -279,6 +344,7 @@ struct ngap_uns {
     // Krylov vectors (rows entries each)
     double *b = nullptr, *x = nullptr, *r = nullptr, *vin = nullptr, *vout = nullptr, *p = nullptr, *q = nullptr,
            *s = nullptr, *t = nullptr, *v = nullptr, *rh = nullptr, *ph = nullptr, *sh = nullptr;
+    float *vin32 = nullptr, *vout32 = nullptr;  // the FP32 cycle's input and output
     double *V = nullptr, *Z = nullptr;  // GMRES bases
     double *grhs = nullptr, *gsol = nullptr;  // ghosted staging for the host entry point
     double *part = nullptr, *scal = nullptr, *h_scal = nullptr;
@@ -309,7 +375,7 @@ int upload(ngap_uns *h, T **dst, const std::vector<T> &src) {
     return 0;
 }
 
-int upload_csr(ngap_uns *h, DCsr &D, const ngap::amg::Csr &M) {
+int upload_csr(ngap_uns *h, DCsr &D, const ngap::amg::Csr &M, bool f32 = false) {
     D.n = M.n;
     D.ncol = M.ncol;
     D.nnz = (long long)M.col.size();
@@ -321,6 +387,23 @@ int upload_csr(ngap_uns *h, DCsr &D, const ngap::amg::Csr &M) {
     RCHK(upload(h, &D.rp, M.rp));
     RCHK(upload(h, &D.col, M.col));
     RCHK(upload(h, &D.val, M.val));
+    if (f32) RCHK(upload(h, &D.val32, std::vector<float>(M.val.begin(), M.val.end())));
+    return 0;
+}
+
+// the vectors and 1/d of one level in precision T
+template <typename T>
+int level_vectors(ngap_uns *h, LVec<T> &V, const std::vector<double> &d, int l) {
+    const int n = (int)d.size();
+    std::vector<T> di(n);
+    for (int i = 0; i < n; i++) di[i] = d[i] > 0.0 ? (T)(1.0 / d[i]) : T(0);
+    RCHK(upload(h, &V.dinv, di));
+    if (l > 0) {
+        RCHK(dalloc(h, (void **)&V.f, sizeof(T) * n));
+        RCHK(dalloc(h, (void **)&V.x, sizeof(T) * n));
+    }
+    RCHK(dalloc(h, (void **)&V.w, sizeof(T) * n));
+    RCHK(dalloc(h, (void **)&V.r, sizeof(T) * n));
     return 0;
 }
 
@@ -333,6 +416,7 @@ void free_device(ngap_uns *h) {
     h->h_scal = nullptr;
     h->L.clear();
     h->Ainv = nullptr;
+    h->vin32 = h->vout32 = nullptr;
     h->setup_done = 0;
 }
 
@@ -346,20 +430,22 @@ bool is_amg(int m) {
 // so after 2 nu - 1 switches the result lies in x (vout on level 0). With one
 // sweep on a sparse level the first one (x = f / d) is never stored: the
 // residual reads f / d, and the prolongation writes f / d + P e into w.
-void vcycle(ngap_uns *h) {
+template <typename T>
+void vcycle(ngap_uns *h, const T *vin, T *vout) {
     hipStream_t s = h->stream;
     const int nl = (int)h->L.size(), nu = h->o.nu;
     if (nl == 1) {
         const int n = h->L[0].A.n;
-        hipLaunchKernelGGL(k_dense, dim3((n + 3) / 4), dim3(256), 0, s, n, h->Ainv, h->vin, h->vout);
+        hipLaunchKernelGGL(k_dense<T>, dim3((n + 3) / 4), dim3(256), 0, s, n, h->Ainv, vin, vout);
         return;
     }
-    std::vector<double *> cur(nl);
+    std::vector<T *> cur(nl);
     for (int l = 0; l < nl - 1; l++) {
         DLevel &D = h->L[l];
-        const double *f = l == 0 ? h->vin : D.f;
-        double *xf = l == 0 ? h->vout : D.x;
-        double *bufs[2] = {D.w, xf};
+        LVec<T> &V = D.vec(T());
+        const T *f = l == 0 ? vin : V.f;
+        T *xf = l == 0 ? vout : V.x;
+        T *bufs[2] = {V.w, xf};
         // fused only on sparse levels: with long rows the two gathers (f and
         // 1/d) cost more than the stored x (measured at 128^3: 110 us fused
         // against 86 us unfused at 30 nonzeros per row, 55 against 59 at 7)
@@ -367,51 +453,69 @@ void vcycle(ngap_uns *h) {
             // the sweep from zero, x = f / d, stays implicit: the residual and
             // the prolongation read f / d
             cur[l] = nullptr;
-            csr_resid0(s, D.A, f, D.dinv, D.r);
+            csr_resid0<T>(s, D.A, f, V.dinv, V.r);
         } else {
             int nb = 0;
-            hipLaunchKernelGGL(k_scale, dim3(vec_blocks(D.A.n)), dim3(256), 0, s, D.A.n, f, D.dinv, bufs[nb]);
+            hipLaunchKernelGGL(k_scale<T>, dim3(vec_blocks(D.A.n)), dim3(256), 0, s, D.A.n, f, (const T *)V.dinv,
+                               bufs[nb]);
             for (int sw = 1; sw < nu; sw++) {
-                csr_jacobi(s, D.A, f, D.dinv, bufs[nb], bufs[nb ^ 1]);
+                csr_jacobi<T>(s, D.A, f, V.dinv, bufs[nb], bufs[nb ^ 1]);
                 nb ^= 1;
             }
             cur[l] = bufs[nb];
-            csr_resid(s, D.A, f, cur[l], D.r);
+            csr_resid<T>(s, D.A, f, cur[l], V.r);
         }
-        csr_mul(s, D.R, D.r, h->L[l + 1].f);
+        csr_mul<T>(s, D.R, V.r, h->L[l + 1].vec(T()).f);
     }
     {
         DLevel &C = h->L[nl - 1];
-        hipLaunchKernelGGL(k_dense, dim3((C.A.n + 3) / 4), dim3(256), 0, s, C.A.n, h->Ainv, C.f, C.x);
+        LVec<T> &V = C.vec(T());
+        hipLaunchKernelGGL(k_dense<T>, dim3((C.A.n + 3) / 4), dim3(256), 0, s, C.A.n, h->Ainv, (const T *)V.f, V.x);
     }
     for (int l = nl - 2; l >= 0; l--) {
         DLevel &D = h->L[l];
-        const double *f = l == 0 ? h->vin : D.f;
-        double *xf = l == 0 ? h->vout : D.x;
-        double *c = cur[l];
+        LVec<T> &V = D.vec(T());
+        const T *f = l == 0 ? vin : V.f;
+        T *xf = l == 0 ? vout : V.x;
+        T *c = cur[l];
+        const T *ec = h->L[l + 1].vec(T()).x;
         if (!c) {
-            c = D.w;
-            csr_add0(s, D.P, f, D.dinv, h->L[l + 1].x, c);
+            c = V.w;
+            csr_add0<T>(s, D.P, f, V.dinv, ec, c);
         } else {
-            csr_add(s, D.P, h->L[l + 1].x, c);
+            csr_add<T>(s, D.P, ec, c);
         }
         for (int sw = 0; sw < nu; sw++) {
-            double *nxt = c == xf ? D.w : xf;
-            csr_jacobi(s, D.A, f, D.dinv, c, nxt);
+            T *nxt = c == xf ? V.w : xf;
+            csr_jacobi<T>(s, D.A, f, V.dinv, c, nxt);
             c = nxt;
         }
     }
 }
 
+// the cycle on the handle's buffers: FP64 vin -> vout, or the FP32 cycle on
+// vin32 -> vout32
+void vcycle_any(ngap_uns *h) {
+    if (h->o.precision == NGAP_MIXED)
+        vcycle<float>(h, h->vin32, h->vout32);
+    else
+        vcycle<double>(h, h->vin, h->vout);
+}
+
 int run_vcycle(ngap_uns *h) {
     const bool timed = h->time_next;
     if (timed) UCHK(hipEventRecord(h->e0, h->stream));
+    const bool mixed = h->o.precision == NGAP_MIXED;
+    if (mixed)
+        hipLaunchKernelGGL(k_to_f32, dim3(vec_blocks(h->rows)), dim3(256), 0, h->stream, h->rows, h->vin, h->vin32);
     if (h->vgraph) {
         UCHK(hipGraphLaunch(h->vgraph, h->stream));
     } else {
-        vcycle(h);
+        vcycle_any(h);
         UCHK(hipGetLastError());
     }
+    if (mixed)
+        hipLaunchKernelGGL(k_to_f64, dim3(vec_blocks(h->rows)), dim3(256), 0, h->stream, h->rows, h->vout32, h->vout);
     if (timed) {
         UCHK(hipEventRecord(h->e1, h->stream));
         h->time_next = false;
@@ -486,8 +590,14 @@ int run_pcg(ngap_uns *h, double bn, int &it) {
         it++;
         if (std::sqrt(rr) / bn < h->rcvg) break;
         RCHK(precond(h));
-        hipLaunchKernelGGL(k_dot, dim3(NPART), dim3(256), 0, s, n, r, z, h->part);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, s, h->part, NPART, h->scal);
+        if (h->o.precision == NGAP_MIXED && is_amg(h->method)) {
+            // the FP32 cycle is not exactly a fixed symmetric operator: flexible beta
+            hipLaunchKernelGGL(k_dot2, dim3(NPART), dim3(256), 0, s, n, r, z, h->q, h->part);
+            hipLaunchKernelGGL(k_fin_beta_flex, dim3(1), dim3(256), 0, s, h->part, NPART, h->scal);
+        } else {
+            hipLaunchKernelGGL(k_dot, dim3(NPART), dim3(256), 0, s, n, r, z, h->part);
+            hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(256), 0, s, h->part, NPART, h->scal);
+        }
         hipLaunchKernelGGL(k_pcg_p, dim3(nb), dim3(256), 0, s, n, h->scal, z, h->p);
     }
     return 0;
@@ -739,7 +849,8 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     ngap_uns_opts op;
     ngap_uns_default_opts(&op);
     if (o) op = *o;
-    if (op.nu < 1 || op.kdim < 1 || op.kdim > 30 || op.max_coarse < 1 || op.max_levels < 1 || op.pmax < 0)
+    if (op.nu < 1 || op.kdim < 1 || op.kdim > 30 || op.max_coarse < 1 || op.max_levels < 1 || op.pmax < 0 ||
+        (op.precision != NGAP_FP64 && op.precision != NGAP_MIXED))
         return NGAP_ERR_ARG;
     if (h->rows < 1) return NGAP_ERR_ARG;
     const double t0 = now_ms();
@@ -791,25 +902,20 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     const int nl = (int)Hh.A.size();
     h->L.resize(nl);
     long long nnz_all = 0;
+    const bool mixed = is_amg(h->method) && op.precision == NGAP_MIXED;
     for (int l = 0; l < nl; l++) {
         DLevel &D = h->L[l];
-        RCHK(upload_csr(h, D.A, Hh.A[l]));
+        RCHK(upload_csr(h, D.A, Hh.A[l], mixed));
         nnz_all += D.A.nnz;
-        const int n = Hh.A[l].n;
         if (is_amg(h->method)) {
-            std::vector<double> di(n);
-            for (int i = 0; i < n; i++) di[i] = Hh.d[l][i] > 0.0 ? 1.0 / Hh.d[l][i] : 0.0;
-            RCHK(upload(h, &D.dinv, di));
             if (l < nl - 1) {
-                RCHK(upload_csr(h, D.P, Hh.P[l]));
-                RCHK(upload_csr(h, D.R, ngap::amg::transpose(Hh.P[l])));
+                RCHK(upload_csr(h, D.P, Hh.P[l], mixed));
+                RCHK(upload_csr(h, D.R, ngap::amg::transpose(Hh.P[l]), mixed));
             }
-            if (l > 0) {
-                RCHK(dalloc(h, (void **)&D.f, sizeof(double) * n));
-                RCHK(dalloc(h, (void **)&D.x, sizeof(double) * n));
-            }
-            RCHK(dalloc(h, (void **)&D.w, sizeof(double) * n));
-            RCHK(dalloc(h, (void **)&D.r, sizeof(double) * n));
+            if (mixed)
+                RCHK(level_vectors<float>(h, D.s, Hh.d[l], l));
+            else
+                RCHK(level_vectors<double>(h, D.d, Hh.d[l], l));
         }
     }
     if (is_amg(h->method)) {
@@ -821,6 +927,10 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     const size_t vb = sizeof(double) * (size_t)h->rows;
     double **vecs[] = {&h->b, &h->x, &h->r, &h->vin, &h->vout, &h->p, &h->q, &h->s, &h->t, &h->v, &h->rh, &h->ph, &h->sh};
     for (double **pv : vecs) RCHK(dalloc(h, (void **)pv, vb));
+    if (mixed) {
+        RCHK(dalloc(h, (void **)&h->vin32, sizeof(float) * (size_t)h->rows));
+        RCHK(dalloc(h, (void **)&h->vout32, sizeof(float) * (size_t)h->rows));
+    }
     if (h->method == NGAP_UNS_GMRES || h->method == NGAP_UNS_GMRES_AMG) {
         RCHK(dalloc(h, (void **)&h->V, vb * (op.kdim + 1)));
         RCHK(dalloc(h, (void **)&h->Z, vb * op.kdim));
@@ -834,7 +944,7 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     if (is_amg(h->method) && op.graphs) {
         hipGraph_t gr = nullptr;
         UCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        vcycle(h);
+        vcycle_any(h);
         UCHK(hipStreamEndCapture(h->stream, &gr));
         const hipError_t e = hipGraphInstantiate(&h->vgraph, gr, nullptr, nullptr, 0);
         (void)hipGraphDestroy(gr);

@@ -52,7 +52,7 @@ module ngap_c
 
    type, bind(C), public :: ngap_uns_opts_t
       real(c_double) :: strong_threshold,max_row_sum
-      integer(c_int) :: pmax,max_coarse,max_levels,nu,kdim,graphs,verbose
+      integer(c_int) :: pmax,max_coarse,max_levels,nu,kdim,graphs,verbose,precision
    end type ngap_uns_opts_t
 
    interface

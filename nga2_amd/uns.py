@@ -15,6 +15,8 @@ writes the participating cells of sol and syncs the ghosts (:485-497); other
 cells keep their values.
 
 DeviceUns is the same solve on torch device tensors (the _device entry point).
+Options (NgapUns.opts, DeviceUns keywords) are the fields of ngap_uns_opts;
+precision=1 runs the multigrid cycle in FP32 under the FP64 iteration.
 """
 from __future__ import annotations
 
@@ -49,7 +51,7 @@ _L = None
 class UnsOpts(C.Structure):
     _fields_ = [("strong_threshold", C.c_double), ("max_row_sum", C.c_double), ("pmax", C.c_int),
                 ("max_coarse", C.c_int), ("max_levels", C.c_int), ("nu", C.c_int), ("kdim", C.c_int),
-                ("graphs", C.c_int), ("verbose", C.c_int)]
+                ("graphs", C.c_int), ("verbose", C.c_int), ("precision", C.c_int)]
 
 
 class UnsStats(C.Structure):

@@ -102,6 +102,48 @@ def test_methods_match_oracle(name, method):
     s.destroy()
 
 
+def test_mixed_vcycle_matches_oracle():
+    """The FP32 cycle (precision NGAP_MIXED) against the oracle's FP64 cycle on
+    the same hierarchy, to FP32 rounding."""
+    print("\n[uns mixed vcycle]", flush=True)
+    P = CASES["masked12"]()
+    ind, A, H = _oracle(P)
+    ds = uns.DeviceUns(P.cfg, P.opr, method=uns.PCG_AMG, max_coarse=MC, precision=1)
+    b = np.random.default_rng(12).standard_normal(A.n)
+    ref = R.vcycle(H, b)
+    d_x = torch.zeros(A.n, dtype=torch.float64, device=DEV)
+    ds.vcycle(torch.from_numpy(b).to(DEV), d_x)
+    got = d_x.cpu().numpy()
+    assert np.abs(got - ref).max() <= 2e-5 * np.abs(ref).max(), np.abs(got - ref).max() / np.abs(ref).max()
+    ds.destroy()
+
+
+@pytest.mark.parametrize("method", [uns.AMG, uns.PCG_AMG, uns.GMRES_AMG, uns.BICGSTAB_AMG])
+@pytest.mark.parametrize("name", ["masked12", "channel8"])
+def test_mixed_precision_methods(name, method):
+    """FP32 cycle under the FP64 iteration: the same tolerance on the true
+    residual, the FP64 oracle's solution to 1e-8, at most three more
+    iterations than the oracle's FP64 count."""
+    print(f"\n[uns mixed {uns.METHOD_NAMES[method]} {name}]", flush=True)
+    P = CASES[name]()
+    cfg = P.cfg
+    ind, A, H = _oracle(P)
+    it_ = cfg.interior
+    act = ind[it_] >= 0
+    b = P.rhs[it_].ravel(order="F")[act.ravel(order="F")]
+    xr, itr, _ = R.solve(method, A, H, b, np.zeros(A.n), 300, 1e-10)
+    s = _solver(P, method, precision=1)
+    s.setup()
+    s.rhs[:] = P.rhs
+    s.sol[:] = 0.0
+    s.solve()
+    assert s.rerr < 1e-10 and s.it <= itr + 3, (s.it, itr, s.rerr)
+    got = s.sol[it_].ravel(order="F")[act.ravel(order="F")]
+    d, e = got - got.mean(), xr - xr.mean()
+    assert np.linalg.norm(d - e) <= 1e-8 * np.linalg.norm(e)
+    s.destroy()
+
+
 def test_errors_and_zero_rhs():
     print("\n[uns errors]", flush=True)
     P = CASES["masked12"]()
@@ -168,5 +210,20 @@ def test_masked64_against_structured():
     err = np.linalg.norm(a - b) / np.linalg.norm(b)
     print(f"[uns masked64] structured PCG+PFMG it {p.it}, rel diff {err:.2e}", flush=True)
     assert err <= 1e-8
+    # the FP32 cycle reaches the same solution
+    m = uns.NgapUns(cfg, "Pressure", uns.PCG_AMG, 7)
+    m.stc[:] = STC7
+    m.opr[:] = P.opr
+    m.init()
+    m.maxit, m.rcvg, m.opts = 200, 1e-10, dict(precision=1)
+    m.setup()
+    m.rhs[:] = P.rhs
+    m.sol[:] = 0.0
+    m.solve()
+    c = m.sol[it_][act]
+    c = c - c.mean()
+    print(f"[uns masked64] mixed it {m.it} rerr {m.rerr:.2e}", flush=True)
+    assert m.rerr < 1e-10 and np.linalg.norm(c - b) <= 1e-8 * np.linalg.norm(b)
+    m.destroy()
     p.destroy()
     s.destroy()
