@@ -102,6 +102,30 @@ def test_methods_match_oracle(name, method):
     s.destroy()
 
 
+@pytest.mark.parametrize("method", [uns.AMG, uns.GMRES_AMG, uns.BICGSTAB_AMG, uns.GMRES])
+def test_nonsymmetric_methods_match_oracle(method):
+    """The implicit convection-diffusion operator on the walled channel grid
+    (nonsymmetric, nonsingular): the solution equals the oracle's to 1e-8 with
+    its iteration count +-1."""
+    print(f"\n[uns nonsymmetric {uns.METHOD_NAMES[method]}]", flush=True)
+    from nga2_amd.config import Config, tanh_stretched, uniform
+
+    cfg = Config(uniform(8), tanh_stretched(8), uniform(8), no=1, yper=False, name="implicit")
+    P = inputs.implicit_problem(cfg, visc=0.02)
+    ind, A, H = _oracle(P)
+    b = P.rhs[cfg.interior].ravel(order="F")
+    xr, itr, _ = R.solve(method, A, H, b, np.zeros(A.n), 300, 1e-10)
+    s = _solver(P, method)
+    s.setup()
+    s.rhs[:] = P.rhs
+    s.sol[:] = 0.0
+    s.solve()
+    got = s.sol[cfg.interior].ravel(order="F")
+    assert s.rerr < 1e-10 and abs(s.it - itr) <= 1, (s.it, itr, s.rerr)
+    assert np.linalg.norm(got - xr) <= 1e-8 * np.linalg.norm(xr)
+    s.destroy()
+
+
 def test_mixed_vcycle_matches_oracle():
     """The FP32 cycle (precision NGAP_MIXED) against the oracle's FP64 cycle on
     the same hierarchy, to FP32 rounding."""

@@ -35,6 +35,11 @@ def _problem(name):
         return inputs.masked_spheres(n=12, nsph=4, rmin=0.15, rmax=0.25)
     if name == "channel8":
         return inputs.channel_spectral(n=8)
+    if name == "implicit8":
+        from nga2_amd.config import tanh_stretched
+
+        cfg = Config(uniform(8), tanh_stretched(8), uniform(8), no=1, yper=False, name="implicit")
+        return inputs.implicit_problem(cfg, visc=0.02)
     return inputs.periodic_gaussian(n=8)
 
 
@@ -211,7 +216,22 @@ def test_methods_against_direct_solve(method):
         R.solve(R.PCG_PARASAIL, A, H, b, np.zeros(A.n), 10, 1e-10)
 
 
-@pytest.mark.parametrize("name", ["masked12", "channel8", "periodic8"])
+@pytest.mark.parametrize("method", [R.AMG, R.GMRES_AMG, R.BICGSTAB_AMG, R.GMRES])
+def test_nonsymmetric_methods_against_direct_solve(method):
+    """The implicit convection-diffusion operator (nonsymmetric, nonsingular):
+    the methods hypre_uns offers for it reach scipy's direct solution."""
+    P = _problem("implicit8")
+    ind, A = _csr(P)
+    D = A.dense()
+    assert np.abs(D - D.T).max() > 0.1 * np.abs(D).max()
+    H = R.build(A, max_coarse=40)
+    b = P.rhs[P.cfg.interior].ravel(order="F")
+    xd = spl.spsolve(sp.csr_matrix(D), b)
+    x, it, rerr = R.solve(method, A, H, b, np.zeros(A.n), 300, 1e-10)
+    assert rerr < 1e-10 and np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+@pytest.mark.parametrize("name", ["masked12", "channel8", "periodic8", "implicit8"])
 def test_library_host_setup_matches_oracle(name):
     P = _problem(name)
     cfg, opr = P.cfg, P.opr
