@@ -168,6 +168,26 @@ def test_mixed_precision_methods(name, method):
     s.destroy()
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_single_level_hierarchy(precision):
+    """216 rows (<= max_coarse 512): the hierarchy is the fine level alone and
+    the cycle is the dense (pseudo-)inverse, so PCG converges at once."""
+    print(f"\n[uns single level, precision {precision}]", flush=True)
+    P = inputs.periodic_gaussian(n=6)
+    s = uns.NgapUns(P.cfg, "Pressure", uns.PCG_AMG, 7)
+    s.stc[:] = STC7
+    s.opr[:] = P.opr
+    s.init()
+    s.maxit, s.rcvg, s.opts = 50, 1e-10, dict(precision=precision)
+    s.setup()
+    assert s.stats()["nlevels"] == 1 and s.ncell == 216
+    s.rhs[:] = P.rhs
+    s.sol[:] = 0.0
+    s.solve()
+    assert s.rerr < 1e-10 and s.it <= (2 if precision == 0 else 4), (s.it, s.rerr)
+    s.destroy()
+
+
 def test_errors_and_zero_rhs():
     print("\n[uns errors]", flush=True)
     P = CASES["masked12"]()
