@@ -32,7 +32,8 @@ int ngapt_level_coefs(const ngap_t *h, int l, double *c, double *ax, double *ay,
 int ngapt_apply(ngap_t *h, int l, int prec, const double *x, double *y);
 /* one symmetric Gauss-Seidel sweep; order bit 0: colour updated first
  * (0 red, 1 black); bit 1: start from a zero iterate (v ignored on input);
- * bit 2: two sweeps of that order in one launch (the post-smoothing pair) */
+ * bit 2: two sweeps of that order in one launch (the post-smoothing pair);
+ * bits 1 and 2: the first two sweeps from a zero iterate in one launch */
 int ngapt_rbgs(ngap_t *h, int l, int prec, const double *f, double *v, int order);
 int ngapt_resid_restrict(ngap_t *h, int l, int prec, const double *f, const double *v, double *fc);
 int ngapt_prolong_add(ngap_t *h, int l, int prec, const double *vc, double *v);

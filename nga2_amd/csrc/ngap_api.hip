@@ -913,6 +913,12 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
     // against 2 x 131 us on the fine level, but is slower on the smaller
     // levels, whose chunks are too short to amortise its deeper pipeline.
     auto post_launches = [&](const Level &L) { return L.g.N >= PAIR_MIN_CELLS ? npost / 2 + (npost & 1) : npost; };
+    // pre-smoothing on those levels: the first two sweeps (from a zero
+    // iterate) as one pair launch, the rest as in the post-smoothing
+    auto pre_launches = [&](const Level &L) {
+        if (L.g.N < PAIR_MIN_CELLS || npre < 2) return npre;
+        return 1 + (npre - 2) / 2 + ((npre - 2) & 1);
+    };
     const int lb = bottom_start(h);
     const int top = lb < nl ? lb : nl - 1;  // first level below the per-level loop
     std::vector<T *> pre(nl);
@@ -921,15 +927,23 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
         T *fin = l == 0 ? v0 : D.v;
-        const int npl = post_launches(L);
-        T *bufs[2] = {((npre + npl) & 1) ? fin : D.w, ((npre + npl) & 1) ? D.w : fin};
+        const int npl = post_launches(L), prl = pre_launches(L);
+        const bool pairs = L.g.N >= PAIR_MIN_CELLS;
+        T *bufs[2] = {((prl + npl) & 1) ? fin : D.w, ((prl + npl) & 1) ? D.w : fin};
         const T *cur = nullptr;
         int nb = 0;
-        for (int sw = 0; sw < npre; sw++) {
-            if (sw == 0)
+        for (int sw = 0; sw < npre;) {
+            if (pairs && npre - sw >= 2) {
+                // two sweeps in one launch; the first pair starts from zero (cur == nullptr)
+                launch_sweep2<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, bufs[nb], 0);
+                sw += 2;
+            } else if (sw == 0) {
                 launch_smooth0<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, bufs[nb], 0);
-            else
+                sw++;
+            } else {
                 launch_sweep<T>(s, L.g, D.c, D.a0, D.a1, D.a2, f, cur, bufs[nb], 0);
+                sw++;
+            }
             cur = bufs[nb];
             nb ^= 1;
         }
@@ -1723,7 +1737,9 @@ static int t_rbgs(ngap_t *h, int l, const double *f, double *v, int order) {
     int rc = up<T>(h->stream, tmp, f, n, &df);
     if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
     if (rc || !dout) return rc ? rc : NGAP_ERR_ALLOC;
-    if (order & 2)  // first sweep from a zero iterate (the V-cycle's pre-smoothing)
+    if ((order & 6) == 6)  // the first two sweeps from a zero iterate in one launch
+        launch_sweep2<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, (const T *)nullptr, dout, order & 1);
+    else if (order & 2)  // first sweep from a zero iterate (the V-cycle's pre-smoothing)
         launch_smooth0<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dout, order & 1);
     else if (order & 4)  // two sweeps of the same order in one launch (the post-smoothing pair)
         launch_sweep2<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dout, order & 1);

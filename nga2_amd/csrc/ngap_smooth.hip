@@ -561,7 +561,10 @@ __global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c
 // four planes in use. One LDS barrier per step.
 constexpr int R2 = 3;  // recompute ring of the fused pair
 
-template <typename T, int SY>
+// ZERO_IN: vin == 0 (the first two sweeps of the pre-smoothing): h1 is f / c,
+// vin is never read, and the pair equals the zero-iterate sweep followed by a
+// sweep of the same order.
+template <typename T, int SY, bool ZERO_IN = false>
 __global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                    const T *__restrict__ a1, const T *__restrict__ a2,
                                                    const T *__restrict__ f, const T *__restrict__ vin,
@@ -622,10 +625,10 @@ __global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ 
         fq[q] = ld(f, kq);
     }
     for (int q = 0; q < 5; q++) a2q[q] = ld(a2, kwrap(s + 4 - q));
-    vin_m = ld(vin, kwrap(s + 2));
-    vin_p = ld(vin, kwrap(s + 4));
-    T vin_P = ld(vin, kP), a0_P = ld(a0, kP), a1_P = ld(a1, kP);
-    T vo_P = ldo(vin, kP), ao_P = out_owns ? ldo(aout, kP) : T(0);
+    vin_m = ZERO_IN ? T(0) : ld(vin, kwrap(s + 2));
+    vin_p = ZERO_IN ? T(0) : ld(vin, kwrap(s + 4));
+    T vin_P = ZERO_IN ? T(0) : ld(vin, kP), a0_P = ld(a0, kP), a1_P = ld(a1, kP);
+    T vo_P = ZERO_IN ? T(0) : ldo(vin, kP), ao_P = out_owns ? ldo(aout, kP) : T(0);
     for (int q = 0; q < 2; q++) h1[q] = h2[q] = h3[q] = T(0);
     int sa = 0;  // slot of plane P in the 5-deep a0/a1 ring
     for (; s < k1; s++) {
@@ -633,11 +636,11 @@ __global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ 
         // prefetch plane P+1 (and vin, a2 of P+2)
         const int kN = kwrap(P + 1), kN2 = kwrap(P + 2);
         const T c_N = ldc(kN), f_N = ld(f, kN), a0_N = ld(a0, kN), a1_N = ld(a1, kN);
-        const T vo_N = ldo(vin, kN), ao_N = out_owns ? ldo(aout, kN) : T(0);
-        const T vin_N2 = ld(vin, kN2), a2_N2 = ld(a2, kN2);
+        const T vo_N = ZERO_IN ? T(0) : ldo(vin, kN), ao_N = out_owns ? ldo(aout, kN) : T(0);
+        const T vin_N2 = ZERO_IN ? T(0) : ld(vin, kN2), a2_N2 = ld(a2, kN2);
         // publish plane P
         const int sv = P & 1;
-        s_vin[sv][ty][tx] = vin_P;
+        if (!ZERO_IN) s_vin[sv][ty][tx] = vin_P;
         s_a0[sa][ty][tx] = a0_P;
         s_a1[sa][ty][tx] = a1_P;
         lds_barrier();
@@ -664,7 +667,7 @@ __global__ void __launch_bounds__(SX *SY) k_sweep2(Geo g, const T *__restrict__ 
         // h1 on P (every position but the corners; the outer ring reads outside values)
         T n1v = T(0);
         if (valid && kP >= 0 && !corner && colour(kP) == c1)
-            n1v = half(0, sa, s_vin[sv], vin_m, vin_p, a2q[1], a2q[0], true);
+            n1v = ZERO_IN ? fq[0] / cq[0] : half(0, sa, s_vin[sv], vin_m, vin_p, a2q[1], a2q[0], true);
         // h2 on P-1 from h1 (depth >= 1)
         const int k_1 = kwrap(P - 1);
         T n2v = T(0);
@@ -1186,7 +1189,8 @@ void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
 
 // Two sweeps of colour order c1 in one launch (k_sweep2): 58 x 10 output
 // tiles, about 512 workgroups, z chunks of >= 16 planes on large levels (each
-// chunk recomputes 6 planes). vin and vout must differ.
+// chunk recomputes 6 planes). vin and vout must differ; vin == nullptr: the
+// first sweep starts from a zero iterate.
 template <typename T>
 void launch_sweep2(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
                    const T *vin, T *vout, int c1) {
@@ -1199,8 +1203,12 @@ void launch_sweep2(hipStream_t s, const Geo &g, const T *c, const T *a0, const T
     int kc = (int)((g.n2 + gz - 1) / gz);
     if (kc < 1) kc = 1;
     const int nz = (g.n2 + kc - 1) / kc;
-    hipLaunchKernelGGL((k_sweep2<T, SY>), dim3((unsigned)(gx * gy * nz)), dim3(SX, SY), 0, s, g, c, a0, a1, a2, f,
-                       vin, vout, c1, kc, gx, gy);
+    if (vin)
+        hipLaunchKernelGGL((k_sweep2<T, SY, false>), dim3((unsigned)(gx * gy * nz)), dim3(SX, SY), 0, s, g, c, a0, a1,
+                           a2, f, vin, vout, c1, kc, gx, gy);
+    else
+        hipLaunchKernelGGL((k_sweep2<T, SY, true>), dim3((unsigned)(gx * gy * nz)), dim3(SX, SY), 0, s, g, c, a0, a1,
+                           a2, f, vin, vout, c1, kc, gx, gy);
 }
 
 template <typename T>

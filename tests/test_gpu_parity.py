@@ -138,6 +138,24 @@ def test_fused_sweep_pair_is_two_sweeps(case, prec):
             assert np.array_equal(fused, two), (name, l, order, np.abs(fused - two).max())
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_fused_pair_from_zero(case, prec):
+    """The pre-smoothing pair (two sweeps from a zero iterate in one launch,
+    the V-cycle's choice on levels of >= 2^24 cells) equals the zero-iterate
+    sweep followed by a sweep of the same order, bit for bit, on every level."""
+    name, P, o, ps = case
+    hk = ps.hooks()
+    rng = np.random.default_rng(10)
+    for l in range(o.nlevels):
+        n = o.level_size(l)
+        x = rng.standard_normal(n)
+        f = rng.standard_normal(n)
+        for order in (0, 1):
+            two = hk.rbgs(l, f, hk.rbgs(l, f, x, 2 | order, prec=prec), order, prec=prec)
+            fused = hk.rbgs(l, f, x, 6 | order, prec=prec)
+            assert np.array_equal(fused, two), (name, l, order, np.abs(fused - two).max())
+
+
 def test_level_kernels_fp32(case):
     name, P, o, ps = case
     hk = ps.hooks()
