@@ -442,6 +442,8 @@ def main():
     hk = ds_mixed.hooks()
     rb_ms, rb_bytes = hk.time_kernel(_lib.K_RBGS, prec=1, reps=args.roofline_reps)
     sp_ms, sp_bytes = hk.time_kernel(_lib.K_SPMV, prec=0, reps=args.roofline_reps)
+    p2_ms, p2_bytes = hk.time_kernel(_lib.K_SWEEP2, prec=1, reps=args.roofline_reps)
+    p0_ms, p0_bytes = hk.time_kernel(_lib.K_SWEEP2_ZERO, prec=1, reps=args.roofline_reps)
     rb_gbs = rb_bytes / (rb_ms * 1e-3) / 1e9
     sp_gbs = sp_bytes / (sp_ms * 1e-3) / 1e9
     profiled = args.n == 256 and args.workload == "channel_spectral"
@@ -456,6 +458,13 @@ def main():
                 "algorithmic_bytes": rb_bytes,
                 "spmv_fp64": {"achieved": round(sp_gbs, 1), "frac": round(sp_gbs / HBM_PEAK_GBS, 4), "avg_ms": sp_ms,
                               "algorithmic_bytes": sp_bytes, "traffic": sp_traffic}}
+    # The mixed cycle's fine level runs its sweeps as fused pairs: two sweeps per pass over the arrays.
+    # "achieved" counts the bytes of one pass; "sweep_equiv" the bytes two single sweeps would move.
+    for key, ms, nb, sweep_b in (("sweep2_fp32", p2_ms, p2_bytes, 2 * rb_bytes),
+                                 ("sweep2_zero_fp32", p0_ms, p0_bytes, 6.0 / 7.0 * rb_bytes + rb_bytes)):
+        gbs = nb / (ms * 1e-3) / 1e9
+        roofline[key] = {"achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_ms": ms,
+                         "algorithmic_bytes": nb, "sweep_equiv_gbs": round(sweep_b / (ms * 1e-3) / 1e9, 1)}
 
     # CPU baseline + FP64 oracle error
     cpu = None

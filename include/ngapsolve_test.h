@@ -25,6 +25,8 @@ extern "C" {
 #define NGAPT_K_PROLONG 3       /* level 1 -> fine prolongation + correction */
 #define NGAPT_K_UPDATE 4        /* PCG x += alpha p, r -= alpha q, ||r||^2 */
 #define NGAPT_K_EXACT 5         /* residual with the exact 7-array operator */
+#define NGAPT_K_SWEEP2 6        /* fine-level fused pair of sweeps (one launch) */
+#define NGAPT_K_SWEEP2_ZERO 7   /* the same pair from a zero iterate (no v in) */
 
 int ngapt_nlevels(const ngap_t *h);
 int ngapt_level_info(const ngap_t *h, int l, int *dims3, int *cdir);

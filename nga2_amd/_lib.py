@@ -22,7 +22,7 @@ ERR_HIP, ERR_NONFINITE, ERR_BREAKDOWN, ERR_ALLOC, ERR_COARSE, ERR_STAGE, ERR_SIN
 STAGE_GUESS, STAGE_PRECOND = 1, 2
 STATUS_CONVERGED, STATUS_MAXIT = 0, 1
 
-K_SPMV, K_RBGS, K_RESTRICT, K_PROLONG, K_UPDATE, K_EXACT = 0, 1, 2, 3, 4, 5
+K_SPMV, K_RBGS, K_RESTRICT, K_PROLONG, K_UPDATE, K_EXACT, K_SWEEP2, K_SWEEP2_ZERO = 0, 1, 2, 3, 4, 5, 6, 7
 
 EXPORTS = [
     "ngap_default_opts", "ngap_create", "ngap_init", "ngap_setup", "ngap_setup_device", "ngap_solve",

@@ -2127,6 +2127,17 @@ int ngapt_time_kernel(ngap_t *h, int kid, int prec, int reps, double *avg_ms, do
                 else
                     launch_sweep<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r, h->z, L0.d.w, 0);
                 return 0;
+            case NGAPT_K_SWEEP2:
+            case NGAPT_K_SWEEP2_ZERO: {
+                const bool z = kid == NGAPT_K_SWEEP2_ZERO;
+                if (prec)
+                    launch_sweep2<float>(s, L0.g, L0.s.c, L0.s.a0, L0.s.a1, L0.s.a2, h->r32, z ? nullptr : h->z32,
+                                         L0.s.w, 0);
+                else
+                    launch_sweep2<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->r, z ? nullptr : h->z,
+                                          L0.d.w, 0);
+                return 0;
+            }
             case NGAPT_K_RESTRICT: {
                 if (h->nlev < 2) return NGAP_ERR_ARG;
                 Level &C = h->lev[1];
@@ -2165,6 +2176,8 @@ int ngapt_time_kernel(ngap_t *h, int kid, int prec, int reps, double *avg_ms, do
     switch (kid) {
         case NGAPT_K_SPMV: b = 6.0 * w * N; break;
         case NGAPT_K_RBGS: b = 7.0 * w * N; break;  // full sweep: 4 coef + f + v in + v out
+        case NGAPT_K_SWEEP2: b = 7.0 * w * N; break;  // two sweeps, the same arrays once
+        case NGAPT_K_SWEEP2_ZERO: b = 6.0 * w * N; break;  // no v in
         case NGAPT_K_RESTRICT: b = (h->nlev > 1 ? (8.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
         case NGAPT_K_PROLONG: b = (h->nlev > 1 ? (2.0 * w * N + 3.0 * w * h->lev[1].g.N) : 0.0); break;
         case NGAPT_K_UPDATE: b = 6.0 * 8.0 * N; break;

@@ -168,6 +168,36 @@ def test_device_entry_points_match_host():
     ds.destroy()
 
 
+def test_kernel_timing_hook():
+    """The timing entry point behind the bench's roofline: every kernel id
+    runs, reports a positive time and the byte model of its launch, and the
+    unknown id and FP32 on an FP64 setup are refused."""
+    print("\n[kernel timing hook]", flush=True)
+    import torch
+
+    P = inputs.periodic_gaussian(n=32)
+    dev = torch.device("cuda:0")
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    N = 32**3
+    per_cell = {_lib.K_SPMV: 6, _lib.K_RBGS: 7, _lib.K_SWEEP2: 7, _lib.K_SWEEP2_ZERO: 6}
+    for prec, w in ((0, 8), (MIXED, 4)):
+        ds = DeviceSolve(P.cfg)
+        ds.setup(d_opr, maxit=200, rcvg=1e-10, precision=prec)
+        hk = ds.hooks()
+        for kid in (_lib.K_SPMV, _lib.K_RBGS, _lib.K_RESTRICT, _lib.K_PROLONG, _lib.K_UPDATE, _lib.K_EXACT,
+                    _lib.K_SWEEP2, _lib.K_SWEEP2_ZERO):
+            ms, nb = hk.time_kernel(kid, prec=1 if prec else 0, reps=2)
+            assert ms > 0 and nb > 0
+            if kid in per_cell:
+                assert nb == per_cell[kid] * w * N
+        with pytest.raises(NgapError):
+            hk.time_kernel(99, prec=1 if prec else 0, reps=1)
+        if not prec:
+            with pytest.raises(NgapError):
+                hk.time_kernel(_lib.K_RBGS, prec=1, reps=1)
+        ds.destroy()
+
+
 @pytest.mark.timeout(900)
 @pytest.mark.parametrize("which", ["channel_spectral", "spheres_two_phase"])
 def test_full_size_256(which):
