@@ -43,7 +43,8 @@ typedef struct ngap_uns_opts {
     double strong_threshold; /* 0.25 (amg_strong_threshold, :29) */
     double max_row_sum;      /* 0.9 (BoomerAMG default) */
     int pmax;                /* interpolation weights kept per row, 4 (BoomerAMG default) */
-    int max_coarse;          /* coarsening stops at <= this many rows; dense inverse there, 512 */
+    int max_coarse;          /* coarsening stops at <= this many rows; dense inverse there, 512
+                                (at most 4096; NGAP_ERR_COARSE when coarsening stalls above 4096 rows) */
     int max_levels;          /* 25 (BoomerAMG default) */
     int nu;                  /* smoothing sweeps before and after the coarse correction, 1 (:264) */
     int kdim;                /* GMRES restart length, 5 (gmres_kdim, :28) */
@@ -80,7 +81,9 @@ int ngap_uns_create(ngap_uns **h, const ngap_grid *g, int nst, const int *stc, i
 int ngap_uns_init(ngap_uns *h, const double *opr);
 
 /* hypre_uns_setup (:153-426): the matrix from opr and the solver's hierarchy.
- * NGAP_ERR_METHOD for pcg_parasail and gmres_pilut. */
+ * NGAP_ERR_METHOD for pcg_parasail and gmres_pilut; NGAP_ERR_ARG for knobs out
+ * of range or a level with more than 2^31-1 nonzeros; NGAP_ERR_ALLOC when a
+ * host or device allocation fails. */
 int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const ngap_uns_opts *o);
 
 /* hypre_uns_solve (:430-500): rhs and the guess from the participating cells

@@ -18,9 +18,13 @@
 #include "ngap_uns_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <new>
+#include <stdexcept>
+#include <system_error>
 #include <thread>
 
 namespace ngap {
@@ -51,6 +55,17 @@ void par_rows(int n, int nt, F &&body) {
         th.emplace_back([&body, lo, hi, t] { body(lo, hi, t); });
     }
     for (auto &x : th) x.join();
+}
+
+// Row pointers from per-row counts (rp[i + 1] holds row i's count). Columns
+// and values are indexed by int32, so a matrix of more nonzeros is refused.
+void scan_counts(std::vector<int> &rp) {
+    long long s = 0;
+    for (size_t i = 1; i < rp.size(); i++) {
+        s += rp[i];
+        if (s > INT_MAX) throw std::length_error("nonzeros exceed int32 indexing");
+        rp[i] = (int)s;
+    }
 }
 
 double rand01(long long i) {
@@ -121,10 +136,8 @@ Graph strong_graph(const Csr &A, const std::vector<char> &S) {
                 G.drp[i + 1]++;
                 G.irp[A.col[q] + 1]++;
             }
-    for (int i = 0; i < A.n; i++) {
-        G.drp[i + 1] += G.drp[i];
-        G.irp[i + 1] += G.irp[i];
-    }
+    scan_counts(G.drp);
+    scan_counts(G.irp);
     G.dep.resize(G.drp[A.n]);
     G.inf.resize(G.irp[A.n]);
     std::vector<int> pd(G.drp.begin(), G.drp.end() - 1), pi(G.irp.begin(), G.irp.end() - 1);
@@ -347,11 +360,10 @@ Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int p
     P.ncol = nc;
     P.rp.assign(n + 1, 0);
     int row = 0;
+    for (int t = 0; t < nt; t++)
+        for (int l : len[t]) P.rp[++row] = l;
+    scan_counts(P.rp);
     for (int t = 0; t < nt; t++) {
-        for (int l : len[t]) {
-            P.rp[row + 1] = P.rp[row] + l;
-            row++;
-        }
         P.col.insert(P.col.end(), pcol[t].begin(), pcol[t].end());
         P.val.insert(P.val.end(), pval[t].begin(), pval[t].end());
     }
@@ -366,7 +378,7 @@ Csr transpose(const Csr &A) {
     T.ncol = A.n;
     T.rp.assign(T.n + 1, 0);
     for (int c : A.col) T.rp[c + 1]++;
-    for (int i = 0; i < T.n; i++) T.rp[i + 1] += T.rp[i];
+    scan_counts(T.rp);
     T.col.resize(A.col.size());
     T.val.resize(A.col.size());
     std::vector<int> pos(T.rp.begin(), T.rp.end() - 1);
@@ -468,7 +480,7 @@ Csr spgemm(const Csr &A, const Csr &B) {
             r.clear();
         }
     });
-    for (int i = 0; i < A.n; i++) C.rp[i + 1] += C.rp[i];
+    scan_counts(C.rp);
     C.col.resize(C.rp[A.n]);
     C.val.resize(C.rp[A.n]);
     par_rows(A.n, nt, [&](int lo, int hi, int) {
@@ -489,6 +501,7 @@ Csr spgemm(const Csr &A, const Csr &B) {
 
 int coarse_inverse(const Csr &A, std::vector<double> &X, int &singular) {
     const int n = A.n;
+    if (n > MAX_DENSE_ROWS) return NGAP_ERR_COARSE;  // coarsening stopped above the dense limit
     std::vector<double> M((size_t)n * n, 0.0);
     double rs = 0.0, dsum = 0.0;
     for (int i = 0; i < n; i++) {
@@ -572,6 +585,24 @@ int build(Hier &H, Csr A, const ngap_uns_opts &o) {
         H.d.push_back(std::move(d));
     }
     return coarse_inverse(H.A.back(), H.Ainv, H.singular);
+}
+
+int build_checked(Hier &H, Csr A, const ngap_uns_opts &o) {
+    try {
+        return build(H, std::move(A), o);
+    } catch (const std::bad_alloc &) {
+        return NGAP_ERR_ALLOC;
+    } catch (const std::length_error &) {
+        return NGAP_ERR_ARG;
+    } catch (const std::system_error &) {
+        return NGAP_ERR_ALLOC;  // no host thread could be started
+    }
+}
+
+bool opts_valid(const ngap_uns_opts &o) {
+    return o.nu >= 1 && o.kdim >= 1 && o.kdim <= 30 && o.max_coarse >= 1 && o.max_coarse <= MAX_DENSE_ROWS &&
+           o.max_levels >= 1 && o.pmax >= 0 && o.strong_threshold >= 0.0 && o.strong_threshold <= 1.0 &&
+           o.max_row_sum > 0.0 && (o.precision == NGAP_FP64 || o.precision == NGAP_MIXED);
 }
 
 }  // namespace amg
@@ -694,17 +725,25 @@ int ngap_amg_build(ngap_amg **H, int n, const int *rowptr, const int *col, const
     ngap_uns_opts d;
     ngap_uns_default_opts(&d);
     const ngap_uns_opts &op = o ? *o : d;
+    if (!amg::opts_valid(op) || rowptr[0] != 0) return NGAP_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (rowptr[i + 1] < rowptr[i]) return NGAP_ERR_ARG;
+    for (long long q = 0; q < rowptr[n]; q++)
+        if (col[q] < 0 || col[q] >= n) return NGAP_ERR_ARG;
     amg::Csr A;
-    A.n = n;
-    A.ncol = n;
-    A.rp.assign(rowptr, rowptr + n + 1);
-    A.col.assign(col, col + rowptr[n]);
-    A.val.assign(val, val + rowptr[n]);
-    for (int c : A.col)
-        if (c < 0 || c >= n) return NGAP_ERR_ARG;
     auto *h = new (std::nothrow) ngap_amg;
     if (!h) return NGAP_ERR_ALLOC;
-    const int rc = amg::build(h->H, std::move(A), op);
+    try {
+        A.n = n;
+        A.ncol = n;
+        A.rp.assign(rowptr, rowptr + n + 1);
+        A.col.assign(col, col + rowptr[n]);
+        A.val.assign(val, val + rowptr[n]);
+    } catch (const std::bad_alloc &) {
+        delete h;
+        return NGAP_ERR_ALLOC;
+    }
+    const int rc = amg::build_checked(h->H, std::move(A), op);
     if (rc) {
         delete h;
         return rc;

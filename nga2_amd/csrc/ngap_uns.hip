@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "ngap_uns_internal.h"
@@ -786,62 +787,7 @@ int do_solve(ngap_uns *h, int *it, double *rerr, double *aerr) {
     return 0;
 }
 
-}  // namespace
-
-// ------------------------------------------------------------------ C ABI
-extern "C" {
-
-int ngap_uns_create(ngap_uns **hp, const ngap_grid *g, int nst, const int *stc, int method) {
-    if (!hp || !g || !stc || nst < 1 || nst > 64) return NGAP_ERR_ARG;
-    if (method < NGAP_UNS_AMG || method > NGAP_UNS_GMRES_PILUT) return NGAP_ERR_METHOD;
-    const long long nxo = g->imaxo_ - g->imino_ + 1, nyo = g->jmaxo_ - g->jmino_ + 1, nzo = g->kmaxo_ - g->kmino_ + 1;
-    if (g->imax_ - g->imin_ + 1 != g->nx || g->jmax_ - g->jmin_ + 1 != g->ny || g->kmax_ - g->kmin_ + 1 != g->nz)
-        return NGAP_ERR_ARG;  // one rank owns the whole grid
-    if (nxo < 1 || nyo < 1 || nzo < 1 || nxo * nyo * nzo >= (1LL << 31)) return NGAP_ERR_ARG;
-    for (int st = 0; st < nst; st++) {
-        const int di = stc[st], dj = stc[nst + st], dk = stc[2 * nst + st];
-        if (g->imin_ + di < g->imino_ || g->imax_ + di > g->imaxo_ || g->jmin_ + dj < g->jmino_ ||
-            g->jmax_ + dj > g->jmaxo_ || g->kmin_ + dk < g->kmino_ || g->kmax_ + dk > g->kmaxo_)
-            return NGAP_ERR_STENCIL;  // an offset reaches past the ghost layer
-    }
-    auto *h = new (std::nothrow) ngap_uns;
-    if (!h) return NGAP_ERR_ALLOC;
-    h->g = *g;
-    h->nst = nst;
-    h->stc.assign(stc, stc + 3 * nst);
-    h->method = method;
-    h->ntot = nxo * nyo * nzo;
-    ngap_uns_default_opts(&h->o);
-    if (hipStreamCreate(&h->stream) != hipSuccess || hipEventCreate(&h->e0) != hipSuccess ||
-        hipEventCreate(&h->e1) != hipSuccess) {
-        delete h;
-        return NGAP_ERR_HIP;
-    }
-    *hp = h;
-    return 0;
-}
-
-int ngap_uns_init(ngap_uns *h, const double *opr) {
-    if (!h || !opr) return NGAP_ERR_ARG;
-    h->ind.assign(h->ntot, -1);
-    const long long n = ngap::umap(h->g, opr, h->nst, h->ind.data());
-    if (n < 0 || n >= (1LL << 31)) return NGAP_ERR_ARG;
-    h->rows = (int)n;
-    h->map.clear();
-    h->map.reserve(n);
-    const long long nxo = h->g.imaxo_ - h->g.imino_ + 1, nyo = h->g.jmaxo_ - h->g.jmino_ + 1;
-    for (int k = h->g.kmin_; k <= h->g.kmax_; k++)
-        for (int j = h->g.jmin_; j <= h->g.jmax_; j++)
-            for (int i = h->g.imin_; i <= h->g.imax_; i++) {
-                const long long p = (i - h->g.imino_) + nxo * ((j - h->g.jmino_) + nyo * (long long)(k - h->g.kmino_));
-                if (h->ind[p] >= 0) h->map.push_back((int)p);
-            }
-    h->inited = 1;
-    h->setup_done = 0;
-    return 0;
-}
-
-int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const ngap_uns_opts *o) {
+int setup_impl(ngap_uns *h, const double *opr, int maxit, double rcvg, const ngap_uns_opts *o) {
     if (!h || !opr) return NGAP_ERR_ARG;
     if (!h->inited) return NGAP_ERR_NOT_INIT;
     if (h->method == NGAP_UNS_PCG_PARASAIL || h->method == NGAP_UNS_GMRES_PILUT) return NGAP_ERR_METHOD;
@@ -849,9 +795,7 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     ngap_uns_opts op;
     ngap_uns_default_opts(&op);
     if (o) op = *o;
-    if (op.nu < 1 || op.kdim < 1 || op.kdim > 30 || op.max_coarse < 1 || op.max_levels < 1 || op.pmax < 0 ||
-        (op.precision != NGAP_FP64 && op.precision != NGAP_MIXED))
-        return NGAP_ERR_ARG;
+    if (!ngap::amg::opts_valid(op)) return NGAP_ERR_ARG;
     if (h->rows < 1) return NGAP_ERR_ARG;
     const double t0 = now_ms();
     free_device(h);
@@ -886,7 +830,7 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
     const double th = now_ms();
     int rc = 0;
     if (is_amg(h->method)) {
-        rc = ngap::amg::build(H->H, std::move(A), op);
+        rc = ngap::amg::build_checked(H->H, std::move(A), op);
     } else {
         H->H.A.push_back(std::move(A));
         H->H.d.emplace_back();
@@ -965,6 +909,75 @@ int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const
         fprintf(stderr, "[ngap_uns] setup: %d rows, %d levels, operator complexity %.2f, %.1f ms (hierarchy %.1f)\n",
                 h->rows, nl, h->st.op_complexity, h->st.setup_ms, h->st.hier_ms);
     return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+int ngap_uns_create(ngap_uns **hp, const ngap_grid *g, int nst, const int *stc, int method) {
+    if (!hp || !g || !stc || nst < 1 || nst > 64) return NGAP_ERR_ARG;
+    if (method < NGAP_UNS_AMG || method > NGAP_UNS_GMRES_PILUT) return NGAP_ERR_METHOD;
+    const long long nxo = g->imaxo_ - g->imino_ + 1, nyo = g->jmaxo_ - g->jmino_ + 1, nzo = g->kmaxo_ - g->kmino_ + 1;
+    if (g->imax_ - g->imin_ + 1 != g->nx || g->jmax_ - g->jmin_ + 1 != g->ny || g->kmax_ - g->kmin_ + 1 != g->nz)
+        return NGAP_ERR_ARG;  // one rank owns the whole grid
+    if (nxo < 1 || nyo < 1 || nzo < 1 || nxo * nyo * nzo >= (1LL << 31)) return NGAP_ERR_ARG;
+    for (int st = 0; st < nst; st++) {
+        const int di = stc[st], dj = stc[nst + st], dk = stc[2 * nst + st];
+        if (g->imin_ + di < g->imino_ || g->imax_ + di > g->imaxo_ || g->jmin_ + dj < g->jmino_ ||
+            g->jmax_ + dj > g->jmaxo_ || g->kmin_ + dk < g->kmino_ || g->kmax_ + dk > g->kmaxo_)
+            return NGAP_ERR_STENCIL;  // an offset reaches past the ghost layer
+    }
+    auto *h = new (std::nothrow) ngap_uns;
+    if (!h) return NGAP_ERR_ALLOC;
+    h->g = *g;
+    h->nst = nst;
+    h->stc.assign(stc, stc + 3 * nst);
+    h->method = method;
+    h->ntot = nxo * nyo * nzo;
+    ngap_uns_default_opts(&h->o);
+    if (hipStreamCreate(&h->stream) != hipSuccess || hipEventCreate(&h->e0) != hipSuccess ||
+        hipEventCreate(&h->e1) != hipSuccess) {
+        delete h;
+        return NGAP_ERR_HIP;
+    }
+    *hp = h;
+    return 0;
+}
+
+int ngap_uns_init(ngap_uns *h, const double *opr) {
+    if (!h || !opr) return NGAP_ERR_ARG;
+    try {
+        h->ind.assign(h->ntot, -1);
+        h->map.reserve(h->ntot);
+    } catch (const std::bad_alloc &) {
+        return NGAP_ERR_ALLOC;
+    }
+    const long long n = ngap::umap(h->g, opr, h->nst, h->ind.data());
+    if (n < 0 || n >= (1LL << 31)) return NGAP_ERR_ARG;
+    h->rows = (int)n;
+    h->map.clear();
+    const long long nxo = h->g.imaxo_ - h->g.imino_ + 1, nyo = h->g.jmaxo_ - h->g.jmino_ + 1;
+    for (int k = h->g.kmin_; k <= h->g.kmax_; k++)
+        for (int j = h->g.jmin_; j <= h->g.jmax_; j++)
+            for (int i = h->g.imin_; i <= h->g.imax_; i++) {
+                const long long p = (i - h->g.imino_) + nxo * ((j - h->g.jmino_) + nyo * (long long)(k - h->g.kmino_));
+                if (h->ind[p] >= 0) h->map.push_back((int)p);
+            }
+    h->inited = 1;
+    h->setup_done = 0;
+    return 0;
+}
+
+int ngap_uns_setup(ngap_uns *h, const double *opr, int maxit, double rcvg, const ngap_uns_opts *o) {
+    // host allocations (the matrix, the hierarchy) report failure as a code;
+    // the handle is then left without a setup
+    try {
+        return setup_impl(h, opr, maxit, rcvg, o);
+    } catch (const std::bad_alloc &) {
+        return NGAP_ERR_ALLOC;
+    }
 }
 
 int ngap_uns_solve_device(ngap_uns *h, const double *d_rhs, double *d_sol, int *it, double *rerr, double *aerr) {

@@ -25,10 +25,18 @@ struct Hier {
     int singular = 0;
 };
 
+// The coarsest level is inverted as a dense matrix: at most this many rows.
+constexpr int MAX_DENSE_ROWS = 4096;
+
 Csr transpose(const Csr &A);
 Csr spgemm(const Csr &A, const Csr &B);
 int coarse_inverse(const Csr &A, std::vector<double> &X, int &singular);
 int build(Hier &H, Csr A, const ngap_uns_opts &o);
+// build() with host allocation failures, int32 index overflow and thread
+// start failures returned as NGAP_ERR_ALLOC / NGAP_ERR_ARG / NGAP_ERR_ALLOC
+int build_checked(Hier &H, Csr A, const ngap_uns_opts &o);
+// the knobs' ranges (max_coarse <= MAX_DENSE_ROWS among them)
+bool opts_valid(const ngap_uns_opts &o);
 
 }  // namespace amg
 

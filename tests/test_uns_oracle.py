@@ -25,7 +25,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
 from conftest import STC7
-from nga2_amd import inputs, uns
+from nga2_amd import _lib, inputs, uns
 from nga2_amd.config import Config, uniform
 from oracle import amg_ref as R
 
@@ -262,3 +262,30 @@ def test_library_rejects_bad_input():
     o = uns.default_opts()
     assert (o.strong_threshold, o.max_row_sum, o.pmax, o.max_coarse, o.max_levels, o.nu, o.kdim) == \
         (0.25, 0.9, 4, 512, 25, 1, 5)
+
+
+def test_library_rejects_bad_sizes():
+    """Malformed row pointers, knobs out of range, and a hierarchy whose
+    coarsening stops above the dense-coarsest limit (4096 rows) are refused
+    with a code, not computed."""
+    lap = sp.diags([-np.ones(9), 2 * np.ones(10), -np.ones(9)], [-1, 0, 1], format="csr")
+    rp, col, val = lap.indptr, lap.indices, lap.data
+    with pytest.raises(uns.NgapError) as e:
+        uns.Hierarchy.build(rp + 1, col, val)  # rowptr[0] != 0
+    assert e.value.code == _lib.ERR_ARG
+    bad = rp.copy()
+    bad[5] = bad[4] - 1  # decreasing row pointer
+    with pytest.raises(uns.NgapError):
+        uns.Hierarchy.build(bad, col, val)
+    for knob in ({"max_coarse": 4097}, {"max_coarse": 0}, {"strong_threshold": 1.5}, {"max_row_sum": 0.0}):
+        with pytest.raises(uns.NgapError) as e:
+            uns.Hierarchy.build(rp, col, val, **knob)
+        assert e.value.code == _lib.ERR_ARG, knob
+    H = uns.Hierarchy.build(rp, col, val, max_coarse=4096)  # the limit itself is accepted
+    H.free()
+    # a diagonal matrix has no strong connections: no coarse points, so the
+    # coarsest level is the whole matrix, here above the dense limit
+    n = 5000
+    with pytest.raises(uns.NgapError) as e:
+        uns.Hierarchy.build(np.arange(n + 1), np.arange(n), np.full(n, 2.0))
+    assert e.value.code == _lib.ERR_COARSE
