@@ -37,6 +37,9 @@ int ngapt_apply(ngap_t *h, int l, int prec, const double *x, double *y);
  * bit 2: two sweeps of that order in one launch (the post-smoothing pair);
  * bits 1 and 2: the first two sweeps from a zero iterate in one launch */
 int ngapt_rbgs(ngap_t *h, int l, int prec, const double *f, double *v, int order);
+/* residual + restriction, prolongation + correction of level l; prec bit 0:
+ * FP32, bit 1: the one-cell-per-thread transfer kernels on every level (the
+ * solver takes the 16-byte vector form where a level allows it) */
 int ngapt_resid_restrict(ngap_t *h, int l, int prec, const double *f, const double *v, double *fc);
 int ngapt_prolong_add(ngap_t *h, int l, int prec, const double *vc, double *v);
 int ngapt_coarse_solve(ngap_t *h, int prec, const double *f, double *v);

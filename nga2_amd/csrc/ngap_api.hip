@@ -1750,7 +1750,7 @@ static int t_rbgs(ngap_t *h, int l, const double *f, double *v, int order) {
 }
 
 template <typename T>
-static int t_rr(ngap_t *h, int l, const double *f, const double *v, double *fc) {
+static int t_rr(ngap_t *h, int l, const double *f, const double *v, double *fc, bool vec) {
     Tmp tmp;
     Level &L = h->lev[l];
     Level &C = h->lev[l + 1];
@@ -1762,13 +1762,13 @@ static int t_rr(ngap_t *h, int l, const double *f, const double *v, double *fc) 
     if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
     if (rc || !dfc || !dr) return rc ? rc : NGAP_ERR_ALLOC;
     launch_resid<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dr);
-    launch_gather<T>(h->stream, L.g, C.g, L.cdir, dr, DC.pm, DC.pp, dfc);
+    launch_gather<T>(h->stream, L.g, C.g, L.cdir, dr, DC.pm, DC.pp, dfc, vec);
     HIPCHK(hipGetLastError());
     return down<T>(h->stream, tmp, dfc, nc, fc);
 }
 
 template <typename T>
-static int t_pr(ngap_t *h, int l, const double *vc, double *v) {
+static int t_pr(ngap_t *h, int l, const double *vc, double *v, bool vec) {
     Tmp tmp;
     Level &L = h->lev[l];
     Level &C = h->lev[l + 1];
@@ -1778,7 +1778,7 @@ static int t_pr(ngap_t *h, int l, const double *vc, double *v) {
     int rc = up<T>(h->stream, tmp, vc, nc, &dvc);
     if (!rc) rc = up<T>(h->stream, tmp, v, n, &dv);
     if (rc) return rc;
-    launch_prolong_to<T>(h->stream, L.g, C.g, L.cdir, DC.pm, DC.pp, dvc, dv, dv);
+    launch_prolong_to<T>(h->stream, L.g, C.g, L.cdir, DC.pm, DC.pp, dvc, dv, dv, vec);
     HIPCHK(hipGetLastError());
     return down<T>(h->stream, tmp, dv, n, v);
 }
@@ -2049,13 +2049,17 @@ int ngapt_rbgs(ngap_t *h, int l, int prec, const double *f, double *v, int order
 }
 
 int ngapt_resid_restrict(ngap_t *h, int l, int prec, const double *f, const double *v, double *fc) {
+    const bool vec = !(prec & 2);
+    prec &= ~2;
     TEST_GUARD(l + 1);
-    return prec ? t_rr<float>(h, l, f, v, fc) : t_rr<double>(h, l, f, v, fc);
+    return prec ? t_rr<float>(h, l, f, v, fc, vec) : t_rr<double>(h, l, f, v, fc, vec);
 }
 
 int ngapt_prolong_add(ngap_t *h, int l, int prec, const double *vc, double *v) {
+    const bool vec = !(prec & 2);
+    prec &= ~2;
     TEST_GUARD(l + 1);
-    return prec ? t_pr<float>(h, l, vc, v) : t_pr<double>(h, l, vc, v);
+    return prec ? t_pr<float>(h, l, vc, v, vec) : t_pr<double>(h, l, vc, v, vec);
 }
 
 int ngapt_coarse_solve(ngap_t *h, int prec, const double *f, double *v) {

@@ -30,13 +30,14 @@ void launch_smooth0(hipStream_t s, const Geo &g, const T *c, const T *a0, const 
 template <typename T>
 void launch_resid(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
                   const T *x, T *r);
-// fc = P^T r
+// fc = P^T r (vec = false: the one-cell-per-thread kernel on every level)
 template <typename T>
-void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc);
+void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc,
+                   bool vec = true);
 // vout = vin + P vc (vout may be vin)
 template <typename T>
 void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
-                       const T *vin, T *vout);
+                       const T *vin, T *vout, bool vec = true);
 template <typename Tin, typename Tout>
 void launch_dense(hipStream_t s, int n, const double *Ainv, const Tin *f, Tout *v);
 int apply_blocks(const Geo &g, int &kc);

@@ -388,6 +388,45 @@ __global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restr
     fc[pc] = s;
 }
 
+// The same restriction for coarsening in y (D = 1) or z (D = 2) on levels
+// whose x rows hold a multiple of CPL cells: x is not coarsened, so CPL
+// consecutive coarse cells of a thread sit over CPL consecutive fine cells,
+// and every array is read with 16-byte loads. Per cell the arithmetic of
+// k_gather_t, in the same order.
+template <typename T, int CPL, int D>
+__global__ void __launch_bounds__(256) k_gather_v(Geo F, Geo C, const T *__restrict__ r, const T *__restrict__ pm,
+                                                  const T *__restrict__ pp, T *__restrict__ fc) {
+    static_assert(D == 1 || D == 2, "x coarsening has no vector form");
+    typedef Vec<T, CPL> V;
+    const int I = (blockIdx.x * blockDim.x + threadIdx.x) * CPL;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I >= C.n0 || J >= C.n1) return;
+    const int pc = (K * C.n1 + J) * C.n0 + I;
+    const int j = D == 1 ? 2 * J : J, k = D == 2 ? 2 * K : K;
+    const int e = D == 1 ? j : k;
+    const int nd = D == 1 ? F.n1 : F.n2;
+    const int per = D == 1 ? F.p1 : F.p2;
+    const int stride = D == 1 ? F.n0 : F.n0 * F.n1;
+    const int pf = (k * F.n1 + j) * F.n0 + I;
+    const int l = nbm(e, nd, per);
+    V s = vld<T, CPL>(r + pf);
+    if (e + 1 < nd) {
+        const V w = vld<T, CPL>(pm + pc), x = vld<T, CPL>(r + pf + stride);
+#pragma unroll
+        for (int q = 0; q < CPL; q++) s.v[q] += w.v[q] * x.v[q];
+    }
+    if (l >= 0) {
+        const int Jd = D == 1 ? J : K;
+        const int jl = nbm(Jd, D == 1 ? C.n1 : C.n2, D == 1 ? C.p1 : C.p2);
+        const int pl = D == 1 ? (K * C.n1 + jl) * C.n0 + I : (jl * C.n1 + J) * C.n0 + I;
+        const V w = vld<T, CPL>(pp + pl), x = vld<T, CPL>(r + pf + (l - e) * stride);
+#pragma unroll
+        for (int q = 0; q < CPL; q++) s.v[q] += w.v[q] * x.v[q];
+    }
+    *reinterpret_cast<V *>(fc + pc) = s;
+}
+
 // ------------------------------------------------------------------ fused sweep
 // One symmetric-order Gauss-Seidel sweep, out of place:
 //   first colour c1:  v1 = (f - sum a vin) / c        (vin: previous iterate)
@@ -754,6 +793,46 @@ __global__ void __launch_bounds__(256) k_prolong_t(Geo F, Geo C, const T *__rest
     vout[p] = vin[p] + add;
 }
 
+// The same prolongation for coarsening in y (D = 1) or z (D = 2) on levels
+// whose x rows hold a multiple of CPL cells: CPL consecutive fine cells per
+// thread, 16-byte accesses, per cell the arithmetic of k_prolong_t.
+template <typename T, int CPL, int D>
+__global__ void __launch_bounds__(256) k_prolong_v(Geo F, Geo C, const T *__restrict__ pm, const T *__restrict__ pp,
+                                                   const T *__restrict__ vc, const T *vin, T *vout) {
+    static_assert(D == 1 || D == 2, "x coarsening has no vector form");
+    typedef Vec<T, CPL> V;
+    const int i = (blockIdx.x * blockDim.x + threadIdx.x) * CPL;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= F.n0 || j >= F.n1) return;
+    const int p = (k * F.n1 + j) * F.n0 + i;
+    const int fd = D == 1 ? j : k;
+    const int Jd = fd >> 1;
+    const int J = D == 1 ? Jd : j, K = D == 2 ? Jd : k;
+    const int pl = (K * C.n1 + J) * C.n0 + i;
+    V add;
+    if ((fd & 1) == 0) {
+        add = vld<T, CPL>(vc + pl);
+    } else {
+        const V w = vld<T, CPL>(pm + pl), x = vld<T, CPL>(vc + pl);
+#pragma unroll
+        for (int q = 0; q < CPL; q++) add.v[q] = w.v[q] * x.v[q];
+        const int jr = nbp(Jd, D == 1 ? C.n1 : C.n2, D == 1 ? C.p1 : C.p2);
+        if (jr >= 0) {
+            const int pr = D == 1 ? (K * C.n1 + jr) * C.n0 + i : (jr * C.n1 + J) * C.n0 + i;
+            const V w2 = vld<T, CPL>(pp + pl), x2 = vld<T, CPL>(vc + pr);
+#pragma unroll
+            for (int q = 0; q < CPL; q++) add.v[q] += w2.v[q] * x2.v[q];
+        }
+    }
+    // vin may be vout: this thread reads its cells before it writes them
+    const V a = *reinterpret_cast<const V *>(vin + p);
+    V o;
+#pragma unroll
+    for (int q = 0; q < CPL; q++) o.v[q] = a.v[q] + add.v[q];
+    *reinterpret_cast<V *>(vout + p) = o;
+}
+
 // coarsest level: v = Ainv f, one wavefront per row (Ainv symmetric)
 template <typename Tin, typename Tout>
 __global__ void __launch_bounds__(256) k_dense_t(int n, const double *__restrict__ Ainv, const Tin *__restrict__ f,
@@ -1027,6 +1106,14 @@ void tile_dims(const Geo &g, int &tx, int &ty) {
     ty = std::min(256 / tx, pow2ceil(g.n1));
 }
 
+// 256-thread blocks for the vector transfers: x threads own cpl cells each
+static void vec_tile_dims(const Geo &g, int cpl, int &tx, int &ty) {
+    tx = std::min(64, pow2ceil(g.n0 / cpl));
+    ty = std::min(256 / tx, pow2ceil(g.n1));
+}
+
+static bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
+
 // z chunk for the tiled march kernels (256-thread blocks)
 int march_blocks(const Geo &g, int &kc) {
     int tx, ty;
@@ -1141,7 +1228,19 @@ void launch_resid(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
 }
 
 template <typename T>
-void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc) {
+void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc,
+                   bool vec) {
+    constexpr int CPL = 16 / sizeof(T);
+    if (vec && d != 0 && F.n0 % CPL == 0 && aligned16(r) && aligned16(pm) && aligned16(pp) && aligned16(fc)) {
+        int tx, ty;
+        vec_tile_dims(C, CPL, tx, ty);
+        dim3 grid((C.n0 / CPL + tx - 1) / tx, (C.n1 + ty - 1) / ty, C.n2);
+        if (d == 1)
+            hipLaunchKernelGGL((k_gather_v<T, CPL, 1>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
+        else
+            hipLaunchKernelGGL((k_gather_v<T, CPL, 2>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
+        return;
+    }
     int tx, ty;
     tile_dims(C, tx, ty);
     dim3 grid((C.n0 + tx - 1) / tx, (C.n1 + ty - 1) / ty, C.n2);
@@ -1213,7 +1312,19 @@ void launch_sweep2(hipStream_t s, const Geo &g, const T *c, const T *a0, const T
 
 template <typename T>
 void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
-                       const T *vin, T *vout) {
+                       const T *vin, T *vout, bool vec) {
+    constexpr int CPL = 16 / sizeof(T);
+    if (vec && d != 0 && F.n0 % CPL == 0 && aligned16(pm) && aligned16(pp) && aligned16(vc) && aligned16(vin) &&
+        aligned16(vout)) {
+        int tx, ty;
+        vec_tile_dims(F, CPL, tx, ty);
+        dim3 grid((F.n0 / CPL + tx - 1) / tx, (F.n1 + ty - 1) / ty, F.n2);
+        if (d == 1)
+            hipLaunchKernelGGL((k_prolong_v<T, CPL, 1>), grid, dim3(tx, ty), 0, s, F, C, pm, pp, vc, vin, vout);
+        else
+            hipLaunchKernelGGL((k_prolong_v<T, CPL, 2>), grid, dim3(tx, ty), 0, s, F, C, pm, pp, vc, vin, vout);
+        return;
+    }
     int tx, ty;
     tile_dims(F, tx, ty);
     dim3 grid((F.n0 + tx - 1) / tx, (F.n1 + ty - 1) / ty, F.n2);
@@ -1245,9 +1356,9 @@ template void launch_bottom<float>(hipStream_t, const BotArgs<float> &);
     template void launch_resid<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
                                   const T *, T *);                                                                 \
     template void launch_gather<T>(hipStream_t, const Geo &, const Geo &, int, const T *, const T *, const T *,    \
-                                   T *);                                                                           \
+                                   T *, bool);                                                                     \
     template void launch_prolong_to<T>(hipStream_t, const Geo &, const Geo &, int, const T *, const T *,           \
-                                       const T *, const T *, T *);                                                 \
+                                       const T *, const T *, T *, bool);                                           \
     template void launch_dense<T, T>(hipStream_t, int, const double *, const T *, T *);
 INST(double)
 INST(float)

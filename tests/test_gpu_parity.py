@@ -156,6 +156,29 @@ def test_fused_pair_from_zero(case, prec):
             assert np.array_equal(fused, two), (name, l, order, np.abs(fused - two).max())
 
 
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vector_transfers_match_scalar(case, prec):
+    """Restriction and prolongation with CPL cells per thread and 16-byte
+    accesses (y/z coarsening, rows a multiple of CPL) equal the
+    one-cell-per-thread kernels bit for bit, on every level."""
+    name, P, o, ps = case
+    hk = ps.hooks()
+    rng = np.random.default_rng(11)
+    for l in range(o.nlevels - 1):
+        n, nc = o.level_size(l), o.level_size(l + 1)
+        x, f, vc = rng.standard_normal(n), rng.standard_normal(n), rng.standard_normal(nc)
+        a, b = hk.resid_restrict(l, f, x, prec=prec), hk.resid_restrict(l, f, x, prec=prec | 2)
+        assert np.array_equal(a, b), (name, "restrict", l, np.abs(a - b).max())
+        a, b = hk.prolong_add(l, vc, x, prec=prec), hk.prolong_add(l, vc, x, prec=prec | 2)
+        assert np.array_equal(a, b), (name, "prolong", l, np.abs(a - b).max())
+    if prec:
+        # FP32 prolongation against the oracle (the FP64 one is checked above)
+        for l in range(o.nlevels - 1):
+            n, nc = o.level_size(l), o.level_size(l + 1)
+            x, vc = rng.standard_normal(n), rng.standard_normal(nc)
+            assert _close(hk.prolong_add(l, vc, x, prec=1), o.prolong_add(l, vc, x), 2e-6), ("prolong", l)
+
+
 def test_level_kernels_fp32(case):
     name, P, o, ps = case
     hk = ps.hooks()
