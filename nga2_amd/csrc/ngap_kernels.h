@@ -20,6 +20,10 @@ struct Geo {
     long long N;
 };
 
+// a * b + c rounded once, in either precision
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ __forceinline__ int nbm(int i, int n, int per) { return i > 0 ? i - 1 : (per ? n - 1 : -1); }
 __device__ __forceinline__ int nbp(int i, int n, int per) { return i < n - 1 ? i + 1 : (per ? 0 : -1); }
 

@@ -358,10 +358,13 @@ __global__ void __launch_bounds__(256) k_resid_t(Geo g, const T *__restrict__ c,
     }
 }
 
-// fc = P^T r : one thread per coarse cell
+// fc = P^T r : one thread per coarse cell. The transfer kernels take their
+// multiply-adds explicitly (fma_t) with contraction off, so that the scalar
+// and vector forms round alike.
 template <typename T, int D>
 __global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restrict__ r, const T *__restrict__ pm,
                                                   const T *__restrict__ pp, T *__restrict__ fc) {
+#pragma clang fp contract(off)
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     const int J = blockIdx.y * blockDim.y + threadIdx.y;
     const int K = blockIdx.z;
@@ -375,7 +378,7 @@ __global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restr
     const int pf = (k * F.n1 + j) * F.n0 + i;
     const int l = nbm(e, nd, per);
     T s = r[pf];
-    if (e + 1 < nd) s += pm[pc] * r[pf + stride];
+    if (e + 1 < nd) s = fma_t(pm[pc], r[pf + stride], s);
     if (l >= 0) {
         const int Jd = D == 0 ? I : (D == 1 ? J : K);
         const int nc = D == 0 ? C.n0 : (D == 1 ? C.n1 : C.n2);
@@ -383,7 +386,7 @@ __global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restr
         const int jl = nbm(Jd, nc, cper);
         const int pl = D == 0 ? (K * C.n1 + J) * C.n0 + jl : (D == 1 ? (K * C.n1 + jl) * C.n0 + I
                                                                       : (jl * C.n1 + J) * C.n0 + I);
-        s += pp[pl] * r[pf + (l - e) * stride];
+        s = fma_t(pp[pl], r[pf + (l - e) * stride], s);
     }
     fc[pc] = s;
 }
@@ -396,6 +399,7 @@ __global__ void __launch_bounds__(256) k_gather_t(Geo F, Geo C, const T *__restr
 template <typename T, int CPL, int D>
 __global__ void __launch_bounds__(256) k_gather_v(Geo F, Geo C, const T *__restrict__ r, const T *__restrict__ pm,
                                                   const T *__restrict__ pp, T *__restrict__ fc) {
+#pragma clang fp contract(off)
     static_assert(D == 1 || D == 2, "x coarsening has no vector form");
     typedef Vec<T, CPL> V;
     const int I = (blockIdx.x * blockDim.x + threadIdx.x) * CPL;
@@ -414,7 +418,7 @@ __global__ void __launch_bounds__(256) k_gather_v(Geo F, Geo C, const T *__restr
     if (e + 1 < nd) {
         const V w = vld<T, CPL>(pm + pc), x = vld<T, CPL>(r + pf + stride);
 #pragma unroll
-        for (int q = 0; q < CPL; q++) s.v[q] += w.v[q] * x.v[q];
+        for (int q = 0; q < CPL; q++) s.v[q] = fma_t(w.v[q], x.v[q], s.v[q]);
     }
     if (l >= 0) {
         const int Jd = D == 1 ? J : K;
@@ -422,9 +426,54 @@ __global__ void __launch_bounds__(256) k_gather_v(Geo F, Geo C, const T *__restr
         const int pl = D == 1 ? (K * C.n1 + jl) * C.n0 + I : (jl * C.n1 + J) * C.n0 + I;
         const V w = vld<T, CPL>(pp + pl), x = vld<T, CPL>(r + pf + (l - e) * stride);
 #pragma unroll
-        for (int q = 0; q < CPL; q++) s.v[q] += w.v[q] * x.v[q];
+        for (int q = 0; q < CPL; q++) s.v[q] = fma_t(w.v[q], x.v[q], s.v[q]);
     }
     *reinterpret_cast<V *>(fc + pc) = s;
+}
+
+// The restriction for coarsening in x on levels whose rows hold a multiple of
+// 2 CPL fine cells: CPL coarse cells per thread, their 2 CPL fine residuals
+// read with two 16-byte loads, the one fine cell to the left (and its pp
+// weight) with scalar loads. Per cell the arithmetic of k_gather_t<T, 0>.
+template <typename T, int CPL>
+__global__ void __launch_bounds__(256) k_gather_xv(Geo F, Geo C, const T *__restrict__ r, const T *__restrict__ pm,
+                                                   const T *__restrict__ pp, T *__restrict__ fc) {
+#pragma clang fp contract(off)
+    typedef Vec<T, CPL> V;
+    const int I0 = (blockIdx.x * blockDim.x + threadIdx.x) * CPL;
+    const int J = blockIdx.y * blockDim.y + threadIdx.y;
+    const int K = blockIdx.z;
+    if (I0 >= C.n0 || J >= C.n1) return;
+    const int rowc = (K * C.n1 + J) * C.n0, rowf = (K * F.n1 + J) * F.n0;
+    const V r0 = vld<T, CPL>(r + rowf + 2 * I0), r1 = vld<T, CPL>(r + rowf + 2 * I0 + CPL);
+    const V w = vld<T, CPL>(pm + rowc + I0), u = vld<T, CPL>(pp + rowc + I0);
+    T rr[2 * CPL];
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        rr[q] = r0.v[q];
+        rr[CPL + q] = r1.v[q];
+    }
+    V s;
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        const int I = I0 + q, e = 2 * I;
+        T t = rr[2 * q];
+        if (e + 1 < F.n0) t = fma_t(w.v[q], rr[2 * q + 1], t);
+        const int l = nbm(e, F.n0, F.p0);
+        if (l >= 0) {
+            T ppl, rl;
+            if (q > 0) {
+                ppl = u.v[q - 1];
+                rl = rr[2 * q - 1];
+            } else {
+                ppl = pp[rowc + nbm(I, C.n0, C.p0)];
+                rl = r[rowf + l];
+            }
+            t = fma_t(ppl, rl, t);
+        }
+        s.v[q] = t;
+    }
+    *reinterpret_cast<V *>(fc + rowc + I0) = s;
 }
 
 // ------------------------------------------------------------------ fused sweep
@@ -767,6 +816,7 @@ __device__ __forceinline__ int gper_(const Geo &g, int d) { return d == 0 ? g.p0
 template <typename T, int D>
 __global__ void __launch_bounds__(256) k_prolong_t(Geo F, Geo C, const T *__restrict__ pm, const T *__restrict__ pp,
                                                    const T *__restrict__ vc, const T *vin, T *vout) {
+#pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int j = blockIdx.y * blockDim.y + threadIdx.y;
     const int k = blockIdx.z;
@@ -787,7 +837,7 @@ __global__ void __launch_bounds__(256) k_prolong_t(Geo F, Geo C, const T *__rest
         if (jr >= 0) {
             const int pr = D == 0 ? (K * C.n1 + J) * C.n0 + jr : (D == 1 ? (K * C.n1 + jr) * C.n0 + I
                                                                           : (jr * C.n1 + J) * C.n0 + I);
-            add += pp[pl] * vc[pr];
+            add = fma_t(pp[pl], vc[pr], add);
         }
     }
     vout[p] = vin[p] + add;
@@ -799,6 +849,7 @@ __global__ void __launch_bounds__(256) k_prolong_t(Geo F, Geo C, const T *__rest
 template <typename T, int CPL, int D>
 __global__ void __launch_bounds__(256) k_prolong_v(Geo F, Geo C, const T *__restrict__ pm, const T *__restrict__ pp,
                                                    const T *__restrict__ vc, const T *vin, T *vout) {
+#pragma clang fp contract(off)
     static_assert(D == 1 || D == 2, "x coarsening has no vector form");
     typedef Vec<T, CPL> V;
     const int i = (blockIdx.x * blockDim.x + threadIdx.x) * CPL;
@@ -822,7 +873,7 @@ __global__ void __launch_bounds__(256) k_prolong_v(Geo F, Geo C, const T *__rest
             const int pr = D == 1 ? (K * C.n1 + jr) * C.n0 + i : (jr * C.n1 + J) * C.n0 + i;
             const V w2 = vld<T, CPL>(pp + pl), x2 = vld<T, CPL>(vc + pr);
 #pragma unroll
-            for (int q = 0; q < CPL; q++) add.v[q] += w2.v[q] * x2.v[q];
+            for (int q = 0; q < CPL; q++) add.v[q] = fma_t(w2.v[q], x2.v[q], add.v[q]);
         }
     }
     // vin may be vout: this thread reads its cells before it writes them
@@ -830,6 +881,49 @@ __global__ void __launch_bounds__(256) k_prolong_v(Geo F, Geo C, const T *__rest
     V o;
 #pragma unroll
     for (int q = 0; q < CPL; q++) o.v[q] = a.v[q] + add.v[q];
+    *reinterpret_cast<V *>(vout + p) = o;
+}
+
+// The prolongation for coarsening in x on levels whose rows hold a multiple
+// of 2 CPL fine cells: CPL fine cells per thread (16-byte accesses to vin and
+// vout), over CPL / 2 coarse cells and the next one to the right. Per cell
+// the arithmetic of k_prolong_t<T, 0>.
+template <typename T, int CPL>
+__global__ void __launch_bounds__(256) k_prolong_xv(Geo F, Geo C, const T *__restrict__ pm, const T *__restrict__ pp,
+                                                    const T *__restrict__ vc, const T *vin, T *vout) {
+#pragma clang fp contract(off)
+    typedef Vec<T, CPL> V;
+    constexpr int H = CPL / 2;
+    const int i0 = (blockIdx.x * blockDim.x + threadIdx.x) * CPL;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i0 >= F.n0 || j >= F.n1) return;
+    const int rowc = (k * C.n1 + j) * C.n0, p = (k * F.n1 + j) * F.n0 + i0;
+    const int J0 = i0 >> 1;
+    T cv[H + 1], cm[H], cp[H];
+#pragma unroll
+    for (int q = 0; q < H; q++) {
+        cv[q] = vc[rowc + J0 + q];
+        cm[q] = pm[rowc + J0 + q];
+        cp[q] = pp[rowc + J0 + q];
+    }
+    const int jr = nbp(J0 + H - 1, C.n0, C.p0);
+    cv[H] = jr >= 0 ? vc[rowc + jr] : T(0);
+    const V a = *reinterpret_cast<const V *>(vin + p);  // vin may be vout: read before the write below
+    V o;
+#pragma unroll
+    for (int q = 0; q < CPL; q++) {
+        const int h = q >> 1;
+        T add;
+        if ((q & 1) == 0) {
+            add = cv[h];
+        } else {
+            add = cm[h] * cv[h];
+            // the right neighbour exists for every coarse cell but a wall-side last one
+            if (h < H - 1 || jr >= 0) add = fma_t(cp[h], cv[h + 1], add);
+        }
+        o.v[q] = a.v[q] + add;
+    }
     *reinterpret_cast<V *>(vout + p) = o;
 }
 
@@ -1231,6 +1325,13 @@ template <typename T>
 void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc,
                    bool vec) {
     constexpr int CPL = 16 / sizeof(T);
+    if (vec && d == 0 && F.n0 % (2 * CPL) == 0 && aligned16(r) && aligned16(pm) && aligned16(pp) && aligned16(fc)) {
+        int tx, ty;
+        vec_tile_dims(C, CPL, tx, ty);
+        dim3 grid((C.n0 / CPL + tx - 1) / tx, (C.n1 + ty - 1) / ty, C.n2);
+        hipLaunchKernelGGL((k_gather_xv<T, CPL>), grid, dim3(tx, ty), 0, s, F, C, r, pm, pp, fc);
+        return;
+    }
     if (vec && d != 0 && F.n0 % CPL == 0 && aligned16(r) && aligned16(pm) && aligned16(pp) && aligned16(fc)) {
         int tx, ty;
         vec_tile_dims(C, CPL, tx, ty);
@@ -1314,6 +1415,13 @@ template <typename T>
 void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
                        const T *vin, T *vout, bool vec) {
     constexpr int CPL = 16 / sizeof(T);
+    if (vec && d == 0 && F.n0 % (2 * CPL) == 0 && aligned16(vin) && aligned16(vout)) {
+        int tx, ty;
+        vec_tile_dims(F, CPL, tx, ty);
+        dim3 grid((F.n0 / CPL + tx - 1) / tx, (F.n1 + ty - 1) / ty, F.n2);
+        hipLaunchKernelGGL((k_prolong_xv<T, CPL>), grid, dim3(tx, ty), 0, s, F, C, pm, pp, vc, vin, vout);
+        return;
+    }
     if (vec && d != 0 && F.n0 % CPL == 0 && aligned16(pm) && aligned16(pp) && aligned16(vc) && aligned16(vin) &&
         aligned16(vout)) {
         int tx, ty;
