@@ -23,6 +23,8 @@ namespace {
 // ---------------------------------------------------------------- scalar slots
 enum { S_RR = 0, S_BB, S_RHO, S_ALPHA, S_BETA, S_PQ, S_RHO0, S_BAD, S_RRT, S_ASYM, S_STR0, S_STR1, S_STR2, S_NSCAL };
 constexpr int NSCAL_ALLOC = 16;
+// device and pinned host scalars, then 4 int flags in the same allocation
+constexpr size_t SCAL_BYTES = sizeof(double) * NSCAL_ALLOC + sizeof(int) * 4;
 
 struct Ghost {
     long long nxo, nyo;
@@ -582,7 +584,7 @@ struct ngap_t {
     double *scal = nullptr;
     int *flags = nullptr;
     double *h_scal = nullptr;  // pinned
-    int *h_flags = nullptr;    // pinned
+    int *h_flags = nullptr;    // pinned, inside the h_scal allocation
     double *Ainv = nullptr;
     int nc = 0;
     int coarse_dense = 0, coarse_singular = 0;
@@ -702,8 +704,7 @@ void free_all(ngap_t *h) {
     h->gV = h->gZ = h->gm = h->o7 = nullptr;
     h->kdim = 0;
     dfree(h, h->parts, sizeof(double) * 4 * MAX_PARTS);
-    dfree(h, h->scal, sizeof(double) * NSCAL_ALLOC);
-    dfree(h, h->flags, sizeof(int) * 4);
+    dfree(h, h->scal, SCAL_BYTES);  // flags share the allocation
     h->opr = nullptr;
     h->part = nullptr;
     h->rhs_g = h->sol_g = nullptr;
@@ -725,24 +726,23 @@ int ensure_base(ngap_t *h) {
     rc |= dmalloc(h, (void **)&h->r32, sizeof(float) * N);
     rc |= dmalloc(h, (void **)&h->z32, sizeof(float) * N);
     rc |= dmalloc(h, (void **)&h->parts, sizeof(double) * 4 * MAX_PARTS);
-    rc |= dmalloc(h, (void **)&h->scal, sizeof(double) * NSCAL_ALLOC);
-    rc |= dmalloc(h, (void **)&h->flags, sizeof(int) * 4);
+    rc |= dmalloc(h, (void **)&h->scal, SCAL_BYTES);
+    if (h->scal) h->flags = reinterpret_cast<int *>(h->scal + NSCAL_ALLOC);
     if (rc) {
         free_all(h);
         return NGAP_ERR_ALLOC;
     }
     if (!h->h_scal) {
-        HIPCHK(hipHostMalloc((void **)&h->h_scal, sizeof(double) * NSCAL_ALLOC, hipHostMallocDefault));
-        HIPCHK(hipHostMalloc((void **)&h->h_flags, sizeof(int) * 4, hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&h->h_scal, SCAL_BYTES, hipHostMallocDefault));
+        h->h_flags = reinterpret_cast<int *>(h->h_scal + NSCAL_ALLOC);
     }
-    HIPCHK(hipMemsetAsync(h->scal, 0, sizeof(double) * NSCAL_ALLOC, h->stream));
-    HIPCHK(hipMemsetAsync(h->flags, 0, sizeof(int) * 4, h->stream));
+    HIPCHK(hipMemsetAsync(h->scal, 0, SCAL_BYTES, h->stream));
     return 0;
 }
 
 int read_scal(ngap_t *h) {
-    HIPCHK(hipMemcpyAsync(h->h_scal, h->scal, sizeof(double) * NSCAL_ALLOC, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(h->h_flags, h->flags, sizeof(int) * 4, hipMemcpyDeviceToHost, h->stream));
+    // the scalars and the flags behind them, one copy
+    HIPCHK(hipMemcpyAsync(h->h_scal, h->scal, SCAL_BYTES, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
 }
@@ -1986,7 +1986,6 @@ int ngap_destroy(ngap_t *h) {
         if (h->stream) (void)hipStreamSynchronize(h->stream);
         free_all(h);
         if (h->h_scal) (void)hipHostFree(h->h_scal);
-        if (h->h_flags) (void)hipHostFree(h->h_flags);
         if (h->h_gm) (void)hipHostFree(h->h_gm);
         if (h->stream) (void)hipStreamDestroy(h->stream);
     }
