@@ -29,6 +29,10 @@ extern "C" {
 #define NGAPT_K_SWEEP2_ZERO 7   /* the same pair from a zero iterate (no v in) */
 
 int ngapt_nlevels(const ngap_t *h);
+/* Makes the n-th next device allocation of the handle fail with
+ * NGAP_ERR_ALLOC (n = 0: none), so a test can check that a setup which ran
+ * out of memory part-way leaves the handle usable by the next setup. */
+int ngapt_fail_alloc(ngap_t *h, int n);
 int ngapt_level_info(const ngap_t *h, int l, int *dims3, int *cdir);
 int ngapt_level_coefs(const ngap_t *h, int l, double *c, double *ax, double *ay, double *az, double *pm, double *pp);
 int ngapt_apply(ngap_t *h, int l, int prec, const double *x, double *y);

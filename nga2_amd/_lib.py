@@ -29,7 +29,7 @@ EXPORTS = [
     "ngap_solve_device", "ngap_set_tolerance", "ngap_set_stage", "ngap_residual_device", "ngap_apply_device", "ngap_destroy", "ngap_last_stats", "ngap_strerror",
 ]
 TEST_EXPORTS = [
-    "ngapt_nlevels", "ngapt_level_info", "ngapt_level_coefs", "ngapt_apply", "ngapt_rbgs", "ngapt_resid_restrict",
+    "ngapt_nlevels", "ngapt_fail_alloc", "ngapt_level_info", "ngapt_level_coefs", "ngapt_apply", "ngapt_rbgs", "ngapt_resid_restrict",
     "ngapt_prolong_add", "ngapt_coarse_solve", "ngapt_vcycle", "ngapt_apply_exact", "ngapt_dot", "ngapt_time_kernel",
 ]
 
@@ -87,6 +87,7 @@ def lib():
     L.ngap_strerror.argtypes = [C.c_int]
     L.ngap_strerror.restype = C.c_char_p
     L.ngapt_nlevels.argtypes = [_vp]
+    L.ngapt_fail_alloc.argtypes = [_vp, C.c_int]
     L.ngapt_level_info.argtypes = [_vp, C.c_int, _ip, _ip]
     L.ngapt_level_coefs.argtypes = [_vp, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]
     L.ngapt_apply.argtypes = [_vp, C.c_int, C.c_int, _dp, _dp]

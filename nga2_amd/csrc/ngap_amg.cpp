@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <new>
 #include <stdexcept>
 #include <system_error>
@@ -43,18 +44,41 @@ int nthreads_for(long long n) {
     return (int)std::max<long long>(1, std::min<long long>(cap, n / 8192));
 }
 
+// An exception in a worker (bad_alloc from a row's scratch vectors) is kept
+// per thread and rethrown on the caller's thread after every started thread
+// has been joined; a thread that cannot be started (system_error) stops the
+// loop the same way. Either way no joinable std::thread is destroyed, so the
+// caller's catch turns the failure into a return code instead of
+// std::terminate.
 template <typename F>
 void par_rows(int n, int nt, F &&body) {
     if (nt <= 1) {
         body(0, n, 0);
         return;
     }
+    std::vector<std::exception_ptr> err(nt);
     std::vector<std::thread> th;
+    th.reserve(nt);  // emplace_back never reallocates below, so only the thread start can throw
+    std::exception_ptr start_err;
     for (int t = 0; t < nt; t++) {
         const int lo = (int)((long long)n * t / nt), hi = (int)((long long)n * (t + 1) / nt);
-        th.emplace_back([&body, lo, hi, t] { body(lo, hi, t); });
+        try {
+            th.emplace_back([&body, &err, lo, hi, t] {
+                try {
+                    body(lo, hi, t);
+                } catch (...) {
+                    err[t] = std::current_exception();
+                }
+            });
+        } catch (...) {
+            start_err = std::current_exception();
+            break;
+        }
     }
     for (auto &x : th) x.join();
+    if (start_err) std::rethrow_exception(start_err);
+    for (auto &e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 // Row pointers from per-row counts (rp[i + 1] holds row i's count). Columns

@@ -605,6 +605,7 @@ struct ngap_t {
     double *o7 = nullptr;    // the exact operator as seven interior arrays
     double *gm = nullptr;    // device scalars: [0,64) dots, [64,128) second-pass dots, 128 ||w||^2, [192,256) y
     double *h_gm = nullptr;  // pinned copy
+    int fail_alloc = 0;      // > 0: the fail_alloc-th next device allocation fails (ngapt_fail_alloc)
 };
 
 namespace {
@@ -623,6 +624,7 @@ hipError_t g_last_hip = hipSuccess;
 int dmalloc(ngap_t *h, void **ptr, size_t bytes) {
     *ptr = nullptr;
     if (bytes == 0) bytes = 8;
+    if (h->fail_alloc > 0 && --h->fail_alloc == 0) return NGAP_ERR_ALLOC;  // injected by a test
     hipError_t e = hipMalloc(ptr, bytes);
     if (e != hipSuccess) {
         g_last_hip = e;
@@ -1052,18 +1054,26 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     }
     int rc = ensure_base(h);
     if (rc) return rc;
-    if (h->method == NGAP_GMRES_PFMG && h->kdim != op.kdim) {
+    if (h->method == NGAP_GMRES_PFMG && (h->kdim != op.kdim || !h->gV || !h->gZ)) {
         const size_t N = (size_t)h->g0.N;
         dfree(h, h->gV, sizeof(double) * N * (h->kdim + 1));
         dfree(h, h->gZ, sizeof(double) * N * h->kdim);
         h->gV = h->gZ = nullptr;
-        h->kdim = op.kdim;
-        if (dmalloc(h, (void **)&h->gV, sizeof(double) * N * (h->kdim + 1)) ||
-            dmalloc(h, (void **)&h->gZ, sizeof(double) * N * h->kdim))
+        h->kdim = 0;  // set below once both bases exist, so a failed setup never leaves a stale size
+        if (dmalloc(h, (void **)&h->gV, sizeof(double) * N * (op.kdim + 1))) return NGAP_ERR_ALLOC;
+        if (dmalloc(h, (void **)&h->gZ, sizeof(double) * N * op.kdim)) {
+            dfree(h, h->gV, sizeof(double) * N * (op.kdim + 1));
+            h->gV = nullptr;
             return NGAP_ERR_ALLOC;
+        }
+        h->kdim = op.kdim;
+    }
+    if (h->method == NGAP_GMRES_PFMG) {
+        const size_t N = (size_t)h->g0.N;
         if (!h->gm && dmalloc(h, (void **)&h->gm, sizeof(double) * 256)) return NGAP_ERR_ALLOC;
         if (!h->o7 && dmalloc(h, (void **)&h->o7, sizeof(double) * N * 7)) return NGAP_ERR_ALLOC;
         if (!h->h_gm) HIPCHK(hipHostMalloc((void **)&h->h_gm, sizeof(double) * 256, hipHostMallocDefault));
+        if (!h->gV || !h->gZ || !h->gm || !h->o7 || !h->h_gm) return NGAP_ERR_ALLOC;
     }
     hipStream_t s = h->stream;
     size_t obytes = sizeof(double) * (size_t)h->nghost * h->nst;
@@ -1185,6 +1195,23 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     h->st.cells = h->g0.N;
     h->st.device_bytes = h->dev_bytes;
     return 0;
+}
+
+// A device allocation that fails part-way through a setup can leave a level
+// or a GMRES basis with some buffers allocated and others null; a later setup
+// of the same grid would then reuse the level as it is. So after such a
+// failure every device buffer of the handle is released, and the next setup
+// allocates all of them again.
+int setup_checked(ngap_t *h, const double *opr, bool on_device, int maxit, double rcvg, int maxlevel,
+                  const ngap_opts *o) {
+    const int rc = do_setup(h, opr, on_device, maxit, rcvg, maxlevel, o);
+    if (rc == NGAP_ERR_ALLOC && h) {
+        (void)hipSetDevice(h->device);
+        (void)hipStreamSynchronize(h->stream);
+        free_all(h);
+        h->setup_done = false;
+    }
+    return rc;
 }
 
 // ---------------------------------------------------------------- PCG
@@ -1916,11 +1943,11 @@ int ngap_init(ngap_t *h) {
 }
 
 int ngap_setup(ngap_t *h, const double *opr, int maxit, double rcvg, int maxlevel, const ngap_opts *o) {
-    return do_setup(h, opr, false, maxit, rcvg, maxlevel, o);
+    return setup_checked(h, opr, false, maxit, rcvg, maxlevel, o);
 }
 
 int ngap_setup_device(ngap_t *h, const double *d_opr, int maxit, double rcvg, int maxlevel, const ngap_opts *o) {
-    return do_setup(h, d_opr, true, maxit, rcvg, maxlevel, o);
+    return setup_checked(h, d_opr, true, maxit, rcvg, maxlevel, o);
 }
 
 int ngap_solve(ngap_t *h, const double *rhs, double *sol, int *it, double *rerr, double *aerr) {
@@ -2003,6 +2030,12 @@ int ngap_last_stats(const ngap_t *h, ngap_stats *s) {
 // ------------------------------------------------------------ test hooks
 
 int ngapt_nlevels(const ngap_t *h) { return (h && h->setup_done) ? h->nlev : -1; }
+
+int ngapt_fail_alloc(ngap_t *h, int n) {
+    if (!h || n < 0) return NGAP_ERR_ARG;
+    h->fail_alloc = n;
+    return 0;
+}
 
 int ngapt_level_info(const ngap_t *h, int l, int *dims3, int *cdir) {
     if (!h || !h->setup_done || l < 0 || l >= h->nlev || !dims3 || !cdir) return NGAP_ERR_ARG;

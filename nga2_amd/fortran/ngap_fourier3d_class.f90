@@ -133,31 +133,47 @@ contains
       this%setup_done=.false.
    end subroutine ngap_fourier3d_destroy
 
+   !> The one line log and print report: a direct solve has no iterations
+   !> or residual history, so the grid it transforms is what it states
+   subroutine ngap_fourier3d_report(this,line)
+      implicit none
+      class(ngap_fourier3d), intent(in) :: this
+      character(len=*), intent(out) :: line
+      line=''
+      write(line,'("ngap_fourier3d ",a," on ",a,": FFT solve of a circulant stencil on ",i0," x ",i0," x ",i0, &
+      &     " cells, set up ",l1)') trim(this%name),trim(this%cfg%name),this%cfg%nx,this%cfg%ny,this%cfg%nz, &
+      &     this%setup_done
+   end subroutine ngap_fourier3d_report
+
    subroutine ngap_fourier3d_log(this)
       use string,   only: str_long
       use messager, only: log
       implicit none
       class(ngap_fourier3d), intent(in) :: this
-      character(len=str_long) :: message
-      if (this%cfg%amRoot) then
-         write(message,'("MI355X fourier3d solver [",a,"] for config [",a,"]")') trim(this%name),trim(this%cfg%name); call log(message)
-      end if
+      character(len=str_long) :: line
+      if (.not.this%cfg%amRoot) return
+      call ngap_fourier3d_report(this,line)
+      call log(line)
    end subroutine ngap_fourier3d_log
 
    subroutine ngap_fourier3d_print(this)
       use, intrinsic :: iso_fortran_env, only: output_unit
+      use string, only: str_long
       implicit none
       class(ngap_fourier3d), intent(in) :: this
-      if (this%cfg%amRoot) then
-         write(output_unit,'("MI355X fourier3d solver [",a,"] for config [",a,"]")') trim(this%name),trim(this%cfg%name)
-      end if
+      character(len=str_long) :: line
+      if (.not.this%cfg%amRoot) return
+      call ngap_fourier3d_report(this,line)
+      write(output_unit,'(a)') trim(line)
    end subroutine ngap_fourier3d_print
 
    subroutine ngap_fourier3d_print_short(this)
       use, intrinsic :: iso_fortran_env, only: output_unit
       implicit none
       class(ngap_fourier3d), intent(in) :: this
-      if (this%cfg%amRoot) write(output_unit,'("MI355X fourier3d solver [",a16,"] for config [",a16,"]")') trim(this%name),trim(this%cfg%name)
+      if (.not.this%cfg%amRoot) return
+      write(output_unit,'("ngap_fourier3d ",a,": direct solve, ",i0," x ",i0," x ",i0)') trim(this%name), &
+      &     this%cfg%nx,this%cfg%ny,this%cfg%nz
    end subroutine ngap_fourier3d_print_short
 
 end module ngap_fourier3d_class

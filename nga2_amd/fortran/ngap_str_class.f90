@@ -163,39 +163,69 @@ contains
       this%setup_done=.false.
    end subroutine ngap_str_destroy
 
+   !> What log and print report: the last setup (levels, precision, method)
+   !> and the last solve (iterations, the true residual with the exact
+   !> operator beside the recursive one, the C-norm ratio HYPRE's PCG stops
+   !> on, fallbacks, times)
+   subroutine ngap_str_report(this,line,n)
+      implicit none
+      class(ngap_str), intent(in) :: this
+      character(len=*), dimension(:), intent(out) :: line
+      integer, intent(out) :: n
+      character(len=12) :: meth,prec,state
+      meth='pcg_pfmg'
+      if (this%method.eq.gmres_pfmg) meth='gmres_pfmg'
+      prec='fp64'
+      if (this%stats%precision.eq.ngap_mixed) prec='mixed'
+      state='converged'
+      if (this%stats%status.ne.0) state='at maxit'
+      line=''
+      write(line(1),'("ngap_str ",a," on ",a,": ",a," (",a,"), ",i0," multigrid levels")') &
+      &     trim(this%name),trim(this%cfg%name),trim(meth),trim(prec),this%stats%nlevels
+      write(line(2),'("   ",i0," iterations of ",i0,", ",a,"; rcvg ",es9.2)') this%it,this%maxit,trim(state),this%rcvg
+      write(line(3),'("   |b-Ax|/|b| exact operator ",es10.3,", recursive ",es10.3,", C-norm ratio ",es10.3)') &
+      &     this%stats%true_rerr,this%stats%rerr,this%stats%cnorm_ratio
+      write(line(4),'("   |b-Ax| ",es10.3,"; fallback flags ",i0,", residual replacements ",i0)') &
+      &     this%aerr,this%stats%fallback,this%stats%restarts
+      write(line(5),'("   setup ",f0.3," ms, solve ",f0.3," ms")') this%stats%setup_ms,this%stats%solve_ms
+      n=5
+   end subroutine ngap_str_report
+
    subroutine ngap_str_log(this)
       use string,   only: str_long
       use messager, only: log
       implicit none
       class(ngap_str), intent(in) :: this
-      character(len=str_long) :: message
-      if (this%cfg%amRoot) then
-         write(message,'("MI355X pressure solver [",a,"] for config [",a,"]")') trim(this%name),trim(this%cfg%name); call log(message)
-         write(message,'(" >     method = ",i0)') this%method; call log(message)
-         write(message,'(" >   it/maxit = ",i0,"/",i0)') this%it,this%maxit; call log(message)
-         write(message,'(" >  aerr/acvg = ",es12.5,"/",es12.5)') this%aerr,this%acvg; call log(message)
-         write(message,'(" >  rerr/rcvg = ",es12.5,"/",es12.5)') this%rerr,this%rcvg; call log(message)
-      end if
+      character(len=str_long), dimension(5) :: line
+      integer :: n,l
+      if (.not.this%cfg%amRoot) return
+      call ngap_str_report(this,line,n)
+      do l=1,n
+         call log(line(l))
+      end do
    end subroutine ngap_str_log
 
    subroutine ngap_str_print(this)
       use, intrinsic :: iso_fortran_env, only: output_unit
+      use string, only: str_long
       implicit none
       class(ngap_str), intent(in) :: this
-      if (this%cfg%amRoot) then
-         write(output_unit,'("MI355X pressure solver [",a,"] for config [",a,"]")') trim(this%name),trim(this%cfg%name)
-         write(output_unit,'(" >     method = ",i0)') this%method
-         write(output_unit,'(" >   it/maxit = ",i0,"/",i0)') this%it,this%maxit
-         write(output_unit,'(" >  aerr/acvg = ",es12.5,"/",es12.5)') this%aerr,this%acvg
-         write(output_unit,'(" >  rerr/rcvg = ",es12.5,"/",es12.5)') this%rerr,this%rcvg
-      end if
+      character(len=str_long), dimension(5) :: line
+      integer :: n,l
+      if (.not.this%cfg%amRoot) return
+      call ngap_str_report(this,line,n)
+      do l=1,n
+         write(output_unit,'(a)') trim(line(l))
+      end do
    end subroutine ngap_str_print
 
    subroutine ngap_str_print_short(this)
       use, intrinsic :: iso_fortran_env, only: output_unit
       implicit none
       class(ngap_str), intent(in) :: this
-      if (this%cfg%amRoot) write(output_unit,'("MI355X pressure solver [",a16,"] for config [",a16,"] -> it/maxit = ",i3,"/",i3," and rerr/rcvg = ",es12.5,"/",es12.5)') trim(this%name),trim(this%cfg%name),this%it,this%maxit,this%rerr,this%rcvg
+      if (.not.this%cfg%amRoot) return
+      write(output_unit,'("ngap_str ",a,": it ",i0,"/",i0,", |b-Ax|/|b| ",es10.3," (rcvg ",es9.2,"), ",i0," levels")') &
+      &     trim(this%name),this%it,this%maxit,this%stats%true_rerr,this%rcvg,this%stats%nlevels
    end subroutine ngap_str_print_short
 
 end module ngap_str_class

@@ -169,39 +169,64 @@ contains
       this%setup_done=.false.
    end subroutine ngap_uns_destroy_
 
+   !> What log and print report: the method, the participating rows and the
+   !> last solve's iterations and residuals (2-norm stopping rule)
+   subroutine ngap_uns_report(this,line,n)
+      implicit none
+      class(ngap_uns), intent(in) :: this
+      character(len=*), dimension(:), intent(out) :: line
+      integer, intent(out) :: n
+      character(len=14), dimension(8), parameter :: mname=[character(len=14) :: 'amg','pcg_amg','gmres_amg', &
+      &     'bicgstab_amg','pcg','pcg_parasail','gmres','gmres_pilut']
+      character(len=14) :: meth
+      integer(c_long_long) :: rows
+      meth='unknown'
+      if (this%method.ge.1.and.this%method.le.8) meth=mname(this%method)
+      rows=0
+      if (c_associated(this%handle)) rows=ngap_uns_rows(this%handle)
+      line=''
+      write(line(1),'("ngap_uns ",a," on ",a,": ",a,", ",i0," participating cells")') &
+      &     trim(this%name),trim(this%cfg%name),trim(meth),rows
+      write(line(2),'("   ",i0," iterations of ",i0,"; |b-Ax|/|b| ",es10.3," (rcvg ",es9.2,"), |b-Ax| ",es10.3)') &
+      &     this%it,this%maxit,this%rerr,this%rcvg,this%aerr
+      n=2
+   end subroutine ngap_uns_report
+
    subroutine ngap_uns_log(this)
       use string,   only: str_long
       use messager, only: log
       implicit none
       class(ngap_uns), intent(in) :: this
-      character(len=str_long) :: message
-      if (this%cfg%amRoot) then
-         write(message,'("MI355X unstructured solver [",a,"] for config [",a,"]")') trim(this%name),trim(this%cfg%name); call log(message)
-         write(message,'(" >     method = ",i0)') this%method; call log(message)
-         write(message,'(" >   it/maxit = ",i0,"/",i0)') this%it,this%maxit; call log(message)
-         write(message,'(" >  aerr/acvg = ",es12.5,"/",es12.5)') this%aerr,this%acvg; call log(message)
-         write(message,'(" >  rerr/rcvg = ",es12.5,"/",es12.5)') this%rerr,this%rcvg; call log(message)
-      end if
+      character(len=str_long), dimension(2) :: line
+      integer :: n,l
+      if (.not.this%cfg%amRoot) return
+      call ngap_uns_report(this,line,n)
+      do l=1,n
+         call log(line(l))
+      end do
    end subroutine ngap_uns_log
 
    subroutine ngap_uns_print(this)
       use, intrinsic :: iso_fortran_env, only: output_unit
+      use string, only: str_long
       implicit none
       class(ngap_uns), intent(in) :: this
-      if (this%cfg%amRoot) then
-         write(output_unit,'("MI355X unstructured solver [",a,"] for config [",a,"]")') trim(this%name),trim(this%cfg%name)
-         write(output_unit,'(" >     method = ",i0)') this%method
-         write(output_unit,'(" >   it/maxit = ",i0,"/",i0)') this%it,this%maxit
-         write(output_unit,'(" >  aerr/acvg = ",es12.5,"/",es12.5)') this%aerr,this%acvg
-         write(output_unit,'(" >  rerr/rcvg = ",es12.5,"/",es12.5)') this%rerr,this%rcvg
-      end if
+      character(len=str_long), dimension(2) :: line
+      integer :: n,l
+      if (.not.this%cfg%amRoot) return
+      call ngap_uns_report(this,line,n)
+      do l=1,n
+         write(output_unit,'(a)') trim(line(l))
+      end do
    end subroutine ngap_uns_print
 
    subroutine ngap_uns_print_short(this)
       use, intrinsic :: iso_fortran_env, only: output_unit
       implicit none
       class(ngap_uns), intent(in) :: this
-      if (this%cfg%amRoot) write(output_unit,'("MI355X unstructured solver [",a16,"] for config [",a16,"] -> it/maxit = ",i3,"/",i3," and rerr/rcvg = ",es12.5,"/",es12.5)') trim(this%name),trim(this%cfg%name),this%it,this%maxit,this%rerr,this%rcvg
+      if (.not.this%cfg%amRoot) return
+      write(output_unit,'("ngap_uns ",a,": it ",i0,"/",i0,", |b-Ax|/|b| ",es10.3," (rcvg ",es9.2,")")') &
+      &     trim(this%name),this%it,this%maxit,this%rerr,this%rcvg
    end subroutine ngap_uns_print_short
 
 end module ngap_uns_class

@@ -15,6 +15,11 @@ Reference:
   src/constant_density/incomp_class.f90:1144-1159 get_div
   src/constant_density/incomp_class.f90:1163-1184 get_pgrad
   src/constant_density/incomp_class.f90:1720-1740 shift_p (tpns_class.f90:2271-2294)
+  src/two_phase/tpns_class.f90:1616-1691          add_surface_tension_jump (CSF)
+  src/two_phase/tpns_class.f90:1695-1716          get_pgrad with the jump
+  src/constant_density/incomp_class.f90:1577-1716 get_mfr, correct_mfr (tpns_class.f90:2131-2267)
+  examples/channel/src/simulation.f90:318-328     rhs scaling, velocity correction
+  examples/falling_drop/src/simulation.f90:302-311 the same with face densities
 """
 from __future__ import annotations
 
@@ -248,3 +253,112 @@ def convection_diffusion(met: Metrics, faces, rho, visc, dt):
     for s in range(6):
         opr[1 + s][it] = offd[s][it]
     return opr
+
+
+def rhs_scaled(cfg: Config, div, rho, dt):
+    """psolv%rhs = -cfg%vol*div*rho/dt over every cell: Fortran evaluates the
+    product left to right and negates the result
+    (examples/channel/src/simulation.f90:318; falling_drop:302 is rho = 1)."""
+    return -(((np.asarray(cfg.vol) * div) * rho) / dt)
+
+
+def correct_velocity(dt, rho, grads, vel, rho_faces=None, P=None, sol=None):
+    """U -= dt*Px/rho (constant density, channel/src/simulation.f90:325-327)
+    or /rho_U (face densities, falling_drop/src/simulation.f90:308-310) over
+    every cell; P += sol (:324, :307). Returns the new (U, V, W) and P."""
+    out = []
+    for ax in range(3):
+        r = rho if rho_faces is None else rho_faces[ax]
+        out.append(vel[ax] - (dt * grads[ax]) / r)
+    Pn = None if P is None else P + sol
+    return out, Pn
+
+
+def get_pgrad_jump(met: Metrics, P, dPj):
+    """tpns get_pgrad: Pgrad_x = sum(divu_x*P(i-1:i)) - dPjx on interior
+    cells, then synced (tpns_class.f90:1695-1716)."""
+    cfg = met.cfg
+    it = cfg.interior
+    out = []
+    for ax in range(3):
+        u0, u1 = met.divu[ax]
+        g = cfg.field()
+        val = (u0 * _shift(P, ax, -1) + u1 * P) - dPj[ax]
+        g[it] = val[it]
+        cfg.sync(g)
+        out.append(g)
+    return out
+
+
+def csf_jump(met: Metrics, sigma, dt, SD, curv, VF, rho_faces, Pj, div):
+    """add_surface_tension_jump without a contact model (tpns_class.f90:
+    1616-1691). Pj: the previous jump (three face arrays). Returns the new
+    jump, its increment DPj (new - previous, every cell) and the new div
+    (interior cells += dt * sum(divp * DPj / rho_face))."""
+    cfg = met.cfg
+    no = cfg.no
+    vol = np.asarray(cfg.vol)
+    n = (cfg.nx, cfg.ny, cfg.nz)
+    region = tuple(slice(no, no + n[a] + 1) for a in range(3))
+    Pn, D = [], []
+    for ax in range(3):
+        u0, u1 = met.divu[ax]
+        SDm, cm, vm, VFm = (_shift(A, ax, -1) for A in (SD, curv, vol, VF))
+        surf = SDm * vm + SD * vol
+        num = (SDm * cm) * vm + (SD * curv) * vol
+        cv = np.zeros_like(surf)
+        pos = surf > 0.0
+        cv[pos] = num[pos] / surf[pos]
+        val = (sigma * cv) * (u0 * VFm + u1 * VF)
+        new = np.array(Pj[ax], dtype=np.float64, order="F", copy=True)
+        new[region] = val[region]
+        Pn.append(new)
+        D.append(new - Pj[ax])
+    terms = []
+    for ax in range(3):
+        p0, p1 = met.divp[ax]
+        r = rho_faces[ax]
+        with np.errstate(divide="ignore", invalid="ignore"):  # the last slab is outside the interior
+            terms.append((p0 * D[ax]) / r + (p1 * _shift(D[ax], ax, 1)) / _shift(r, ax, 1))
+    it = cfg.interior
+    dn = np.array(div, dtype=np.float64, order="F", copy=True)
+    dn[it] = div[it] + dt * ((terms[0][it] + terms[1][it]) + terms[2][it])
+    return Pn, D, dn
+
+
+def correct_mfr(cfg: Config, bconds, rho, vel):
+    """get_mfr + correct_mfr (incomp_class.f90:1577-1716; tpns with rho = 1).
+    bconds: dicts with face (0/1/2), rdir (+1/-1), can_correct, n_in and ijk
+    (Fortran (i, j, k) rows; the first n_in are interior cells). Sums run in
+    the reference's sequential order. Returns (new velocities, mfr_error)."""
+    d = (cfg.dx, cfg.dy, cfg.dz)
+    lo = (cfg.imino, cfg.jmino, cfg.kmino)
+    mfr, carea = [], 0.0
+    for bc in bconds:
+        f = bc["face"]
+        a1, a2 = ((1, 2), (2, 0), (0, 1))[f]
+        F = vel[f]
+        m = 0.0
+        area = 0.0
+        for t in range(bc["n_in"]):
+            ijk = [int(bc["ijk"][t][c]) - lo[c] for c in range(3)]
+            w1, w2 = float(d[a1][ijk[a1]]), float(d[a2][ijk[a2]])
+            m = m + (((bc["rdir"] * rho) * float(F[tuple(ijk)])) * w1) * w2
+            area = area + w1 * w2
+        mfr.append(m)
+        carea = carea + area * (1.0 if bc["can_correct"] else 0.0)
+    err = 0.0
+    for m in mfr:
+        err = err + m
+    out = [np.array(v, dtype=np.float64, order="F", copy=True) for v in vel]
+    tiny = 10.0 * np.finfo(np.float64).eps
+    if abs(err) < tiny or abs(carea) < tiny:
+        return out, err
+    vc = -err / (rho * carea)
+    for bc in bconds:
+        if not bc["can_correct"]:
+            continue
+        for t in range(len(bc["ijk"])):
+            ijk = tuple(int(bc["ijk"][t][c]) - lo[c] for c in range(3))
+            out[bc["face"]][ijk] = out[bc["face"]][ijk] + bc["rdir"] * vc
+    return out, err

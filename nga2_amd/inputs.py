@@ -181,7 +181,7 @@ def sphere_set(nsph, rmin, rmax, seed):
     return centres, radii
 
 
-def sphere_vf(cfg, centres, radii, nsub=4):
+def sphere_vf(cfg, centres, radii, nsub=4, device=None):
     """Liquid volume fraction of the union of spheres (periodic images) on a
     uniform periodic unit-cube grid.
 
@@ -213,7 +213,9 @@ def sphere_vf(cfg, centres, radii, nsub=4):
     cut = np.nonzero(np.abs(G) <= hd)
     ci, cj, ck = cut
     M = len(ci)
-    if M:
+    if M and nsub >= 8 and (nsub & (nsub - 1)) == 0:
+        VF[ci, cj, ck] = _cut_fraction_tree(xc[ci], xc[cj], xc[ck], centres, radii, h, hd, nsub, device=device)
+    elif M:
         off = (np.arange(nsub) + 0.5) / nsub - 0.5
         ox, oy, oz = np.meshgrid(off, off, off, indexing="ij")
         ox, oy, oz = (o.ravel() * h for o in (ox, oy, oz))
@@ -237,6 +239,133 @@ def sphere_vf(cfg, centres, radii, nsub=4):
     return VF
 
 
+def _cut_fraction_tree(ccx, ccy, ccz, centres, radii, h, hd, nsub, device=None):
+    """Fraction of the nsub^3 leaf midpoints of each cut cell that lie inside
+    the union of spheres, found by recursive octree refinement as
+    cube_refine_vol does (src/libraries/mms_geom.f90:554-626): a sub-cube
+    whose level-set value r - |x - c| at its centre exceeds its half-diagonal
+    in magnitude is wholly inside or outside (the level set is 1-Lipschitz),
+    so only cut sub-cubes are split, down to the leaves (log2(nsub) levels).
+    The leaves are classified by their midpoint with the same arithmetic as
+    the uniform sampling of sphere_vf, so the result equals it exactly.
+    Cells are processed in chunks; device: a torch device to run on (the
+    same float64 arithmetic), or None for numpy."""
+    M = len(ccx)
+    # candidate spheres of every cut cell: those whose surface passes within hd
+    # of its centre (the others cannot change the union inside the cell),
+    # padded to K per cell with a sphere of radius 0 far away (never inside,
+    # never the maximum)
+    pc, pdx, pdy, pdz, pr = [], [], [], [], []
+    for c, r in zip(centres, radii):
+        dx = (ccx - c[0] + 0.5) % 1.0 - 0.5
+        dy = (ccy - c[1] + 0.5) % 1.0 - 0.5
+        dz = (ccz - c[2] + 0.5) % 1.0 - 0.5
+        dc = np.sqrt(dx * dx + dy * dy + dz * dz)
+        idx = np.nonzero(np.abs(r - dc) <= hd)[0]
+        pc.append(idx)
+        pdx.append(dx[idx])
+        pdy.append(dy[idx])
+        pdz.append(dz[idx])
+        pr.append(np.full(len(idx), r))
+    pc = np.concatenate(pc)
+    order = np.argsort(pc, kind="stable")
+    pc = pc[order]
+    per = np.bincount(pc, minlength=M)
+    K = max(1, int(per.max()) if M else 1)
+    slot = np.arange(len(pc)) - np.repeat(np.cumsum(per) - per, per)
+    DX = np.full((M, K), 1e3)
+    DY = np.full((M, K), 1e3)
+    DZ = np.full((M, K), 1e3)
+    R = np.zeros((M, K))
+    for A, parts in ((DX, pdx), (DY, pdy), (DZ, pdz), (R, pr)):
+        A[pc, slot] = np.concatenate(parts)[order]
+    if device is not None:
+        import torch
+
+        return _tree_torch(torch, device, DX, DY, DZ, R, h, nsub)
+    out = np.zeros(M)
+    sh = np.array([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], dtype=np.int64)
+    chunk = max(1, (1 << 22) // (nsub * nsub * 8 * K))
+    for c0 in range(0, M, chunk):
+        c1 = min(M, c0 + chunk)
+        dX, dY, dZ, rR = DX[c0:c1], DY[c0:c1], DZ[c0:c1], R[c0:c1]
+        count = np.zeros(c1 - c0, dtype=np.int64)
+        cell = np.arange(c1 - c0)
+        mx = np.zeros(c1 - c0, dtype=np.int64)
+        my = np.zeros(c1 - c0, dtype=np.int64)
+        mz = np.zeros(c1 - c0, dtype=np.int64)
+        w = nsub
+        while len(cell):
+            if w == 1:
+                px = dX[cell] + (((mx + 0.5) / nsub - 0.5) * h)[:, None]
+                py = dY[cell] + (((my + 0.5) / nsub - 0.5) * h)[:, None]
+                pz = dZ[cell] + (((mz + 0.5) / nsub - 0.5) * h)[:, None]
+                ins = np.any((px * px + py * py + pz * pz) < rR[cell] * rR[cell], axis=1)
+                np.add.at(count, cell[ins], 1)
+                break
+            px = dX[cell] + (((mx + 0.5 * w) / nsub - 0.5) * h)[:, None]
+            py = dY[cell] + (((my + 0.5 * w) / nsub - 0.5) * h)[:, None]
+            pz = dZ[cell] + (((mz + 0.5 * w) / nsub - 0.5) * h)[:, None]
+            g = np.max(rR[cell] - np.sqrt(px * px + py * py + pz * pz), axis=1)
+            hw = 0.5 * np.sqrt(3.0) * h * w / nsub * (1.0 + 1e-9)
+            ins = g > hw
+            np.add.at(count, cell[ins], w ** 3)
+            keep = np.abs(g) <= hw
+            cell, mx, my, mz = cell[keep], mx[keep], my[keep], mz[keep]
+            w //= 2
+            n = len(cell)
+            cell = np.repeat(cell, 8)
+            mx = np.repeat(mx, 8) + np.tile(sh[:, 0] * w, n)
+            my = np.repeat(my, 8) + np.tile(sh[:, 1] * w, n)
+            mz = np.repeat(mz, 8) + np.tile(sh[:, 2] * w, n)
+        out[c0:c1] = count / float(nsub ** 3)
+    return out
+
+
+def _tree_torch(torch, device, DX, DY, DZ, R, h, nsub):
+    """_cut_fraction_tree's refinement in torch float64 on a device."""
+    M, K = DX.shape
+    dev = torch.device(device)
+    tX, tY, tZ, tR = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (DX, DY, DZ, R))
+    sh = torch.tensor([[a, b, c] for c in (0, 1) for b in (0, 1) for a in (0, 1)], dtype=torch.int64, device=dev)
+    out = torch.zeros(M, dtype=torch.float64, device=dev)
+    chunk = max(1, (1 << 24) // (nsub * nsub * 8 * K))
+    for c0 in range(0, M, chunk):
+        c1 = min(M, c0 + chunk)
+        dX, dY, dZ, rR = tX[c0:c1], tY[c0:c1], tZ[c0:c1], tR[c0:c1]
+        count = torch.zeros(c1 - c0, dtype=torch.int64, device=dev)
+        cell = torch.arange(c1 - c0, device=dev)
+        mx = torch.zeros(c1 - c0, dtype=torch.int64, device=dev)
+        my = torch.zeros_like(mx)
+        mz = torch.zeros_like(mx)
+        w = nsub
+        while cell.numel():
+            if w == 1:
+                px = dX[cell] + (((mx + 0.5) / nsub - 0.5) * h)[:, None]
+                py = dY[cell] + (((my + 0.5) / nsub - 0.5) * h)[:, None]
+                pz = dZ[cell] + (((mz + 0.5) / nsub - 0.5) * h)[:, None]
+                ins = torch.any((px * px + py * py + pz * pz) < rR[cell] * rR[cell], dim=1)
+                count.index_add_(0, cell[ins], torch.ones_like(cell[ins]))
+                break
+            px = dX[cell] + (((mx + 0.5 * w) / nsub - 0.5) * h)[:, None]
+            py = dY[cell] + (((my + 0.5 * w) / nsub - 0.5) * h)[:, None]
+            pz = dZ[cell] + (((mz + 0.5 * w) / nsub - 0.5) * h)[:, None]
+            g = torch.max(rR[cell] - torch.sqrt(px * px + py * py + pz * pz), dim=1).values
+            hw = 0.5 * np.sqrt(3.0) * h * w / nsub * (1.0 + 1e-9)
+            ins = g > hw
+            count.index_add_(0, cell[ins], torch.full_like(cell[ins], w ** 3))
+            keep = torch.abs(g) <= hw
+            cell, mx, my, mz = cell[keep], mx[keep], my[keep], mz[keep]
+            w //= 2
+            n = cell.numel()
+            cell = cell.repeat_interleave(8)
+            mx = mx.repeat_interleave(8) + (sh[:, 0] * w).repeat(n)
+            my = my.repeat_interleave(8) + (sh[:, 1] * w).repeat(n)
+            mz = mz.repeat_interleave(8) + (sh[:, 2] * w).repeat(n)
+        out[c0:c1] = count.to(torch.float64) / float(nsub ** 3)
+    return out.cpu().numpy()
+
+
 def face_densities(cfg, vf, rho_l, rho_g):
     """Cell density VF rho_l + (1 - VF) rho_g (periodic ghosts synced), then the
     arithmetic mean of the two cells on each face: rho_U(i) = (rho(i-1) + rho(i)) / 2
@@ -258,12 +387,16 @@ def face_densities(cfg, vf, rho_l, rho_g):
 
 
 def spheres_two_phase(n=256, seed=3003, vel_seed=1002, nsph=50, rmin=0.05, rmax=0.15, rho_l=1000.0, rho_g=1.0,
-                      nsub=4, no=1):
+                      nsub=4, no=1, device=None):
+    """nsub = 16 is the four-level refinement of falling_drop (amr_ref_lvl = 4,
+    examples/falling_drop/src/simulation.f90:91,112), the configuration-3
+    recipe (CONFIG3_NSUB); small test grids keep nsub = 4. device: a torch
+    device for the cut-cell refinement (same result)."""
     x = uniform(n)
     cfg = Config(x, x, x, no=no, xper=True, yper=True, zper=True, name="spheres_two_phase")
     met = incomp.Metrics(cfg)
     centres, radii = sphere_set(nsph, rmin, rmax, seed)
-    vf = sphere_vf(cfg, centres, radii, nsub=nsub)
+    vf = sphere_vf(cfg, centres, radii, nsub=nsub, device=device)
     rho_faces = face_densities(cfg, vf, rho_l, rho_g)
     opr = incomp.laplacian(met, rho_faces=rho_faces)
     U, V, W = _gaussian_faces(cfg, _rng(vel_seed))
@@ -327,10 +460,30 @@ class RhsSequence:
         return g.reshape(-1)
 
 
+# the four-level sub-cell refinement of configuration 3's volume fraction
+CONFIG3_NSUB = 16
+
+
+def spheres_two_phase_config3(n=256, **kw):
+    """Configuration 3 as specified: spheres_two_phase with the four-level
+    refinement (nsub = 16), the cut cells refined on the GPU when torch sees
+    one (the same result as on the host, see _cut_fraction_tree)."""
+    device = None
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            device = "cuda:0"
+    except ImportError:
+        pass
+    kw.setdefault("nsub", CONFIG3_NSUB)
+    return spheres_two_phase(n=n, device=device, **kw)
+
+
 WORKLOADS = {
     "periodic_gaussian": periodic_gaussian,
     "channel_spectral": channel_spectral,
-    "spheres_two_phase": spheres_two_phase,
+    "spheres_two_phase": spheres_two_phase_config3,
 }
 
 

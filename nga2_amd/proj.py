@@ -30,12 +30,25 @@ def _bind(L):
     L.ngap_proj_pgrad.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.ngap_proj_shift_p.argtypes = [_vp, _vp, _dp]
     L.ngap_proj_sync.argtypes = [_vp, _vp]
+    L.ngap_proj_rhs.argtypes = [_vp, _vp, C.c_double, C.c_double, _vp]
+    L.ngap_proj_correct.argtypes = [_vp, C.c_double, C.c_double] + [_vp] * 11
+    L.ngap_proj_pgrad_jump.argtypes = [_vp] + [_vp] * 7
+    L.ngap_proj_csf_jump.argtypes = [_vp, C.c_double, C.c_double] + [_vp] * 13
+    L.ngap_proj_set_bconds.argtypes = [_vp, C.c_int, _vp, _dp, _dp, _dp]
+    L.ngap_proj_correct_mfr.argtypes = [_vp, C.c_double, _vp, _vp, _vp, _dp]
     L._proj_bound = True
     return L
 
 
 EXPORTS = ["ngap_proj_create", "ngap_proj_destroy", "ngap_proj_laplacian", "ngap_proj_div", "ngap_proj_pgrad",
-           "ngap_proj_shift_p", "ngap_proj_sync"]
+           "ngap_proj_shift_p", "ngap_proj_sync", "ngap_proj_rhs", "ngap_proj_correct", "ngap_proj_pgrad_jump",
+           "ngap_proj_csf_jump", "ngap_proj_set_bconds", "ngap_proj_correct_mfr"]
+
+
+class Bcond(C.Structure):
+    """ngap_bcond: one boundary condition of correct_mfr."""
+    _fields_ = [("face", C.c_int), ("rdir", C.c_int), ("can_correct", C.c_int), ("n_in", C.c_int),
+                ("n_all", C.c_int), ("ijk", C.POINTER(C.c_int))]
 
 
 def _pairs(a, b):
@@ -93,6 +106,54 @@ class DeviceProjection:
     def sync(self, dA):
         self._torch_sync()
         check("ngap_proj_sync", lib().ngap_proj_sync(self._h, _ptr(dA)))
+
+    def rhs(self, d_div, rho, dt, d_rhs):
+        """d_rhs = -vol*div*rho/dt (rho = 1 for the two-phase form)."""
+        self._torch_sync()
+        check("ngap_proj_rhs", lib().ngap_proj_rhs(self._h, _ptr(d_div), float(rho), float(dt), _ptr(d_rhs)))
+
+    def correct(self, dt, rho, grads, vel, rho_faces=None, P=None, sol=None):
+        """U -= dt*Px/rho (or /rho_U), likewise V, W; P += sol when given."""
+        self._torch_sync()
+        ru, rv, rw = rho_faces if rho_faces is not None else (None, None, None)
+        check("ngap_proj_correct", lib().ngap_proj_correct(
+            self._h, float(dt), float(rho), _ptr(ru), _ptr(rv), _ptr(rw), *[_ptr(g) for g in grads],
+            *[_ptr(v) for v in vel], _ptr(P), _ptr(sol)))
+
+    def pgrad_jump(self, dP, dPj, grads):
+        self._torch_sync()
+        check("ngap_proj_pgrad_jump", lib().ngap_proj_pgrad_jump(self._h, _ptr(dP), *[_ptr(a) for a in dPj],
+                                                                 *[_ptr(g) for g in grads]))
+
+    def csf_jump(self, sigma, dt, SD, curv, VF, rho_faces, Pj, DPj, div):
+        """Pj (in: previous, out: new), DPj (out), div (in/out), all device tensors."""
+        self._torch_sync()
+        check("ngap_proj_csf_jump", lib().ngap_proj_csf_jump(
+            self._h, float(sigma), float(dt), _ptr(SD), _ptr(curv), _ptr(VF), *[_ptr(r) for r in rho_faces],
+            *[_ptr(a) for a in Pj], *[_ptr(a) for a in DPj], _ptr(div)))
+
+    def set_bconds(self, bconds):
+        """bconds: dicts with face, rdir, can_correct, n_in, ijk ((n_all, 3) Fortran indices)."""
+        cfg = self.cfg
+        self._bc_keep = []
+        arr = (Bcond * max(1, len(bconds)))()
+        for t, bc in enumerate(bconds):
+            ijk = np.ascontiguousarray(np.asarray(bc["ijk"], dtype=np.int32).reshape(-1, 3))
+            self._bc_keep.append(ijk)
+            arr[t] = Bcond(int(bc["face"]), int(bc["rdir"]), int(bool(bc["can_correct"])), int(bc["n_in"]),
+                           int(ijk.shape[0]), ijk.ctypes.data_as(C.POINTER(C.c_int)))
+        d = [np.ascontiguousarray(np.asarray(a, dtype=np.float64)) for a in (cfg.dx, cfg.dy, cfg.dz)]
+        self._bc_keep += d
+        check("ngap_proj_set_bconds", lib().ngap_proj_set_bconds(self._h, len(bconds), C.cast(arr, _vp),
+                                                                 *[a.ctypes.data_as(_dp) for a in d]))
+
+    def correct_mfr(self, rho, vel):
+        """correct_mfr on device velocities; returns the flow-rate mismatch."""
+        self._torch_sync()
+        e = C.c_double()
+        check("ngap_proj_correct_mfr", lib().ngap_proj_correct_mfr(self._h, float(rho), *[_ptr(v) for v in vel],
+                                                                   C.byref(e)))
+        return e.value
 
     def destroy(self):
         if self._h:

@@ -2,6 +2,7 @@
 sphere VF and of the face densities give the numpy generators' values bit for
 bit (run with torch on the CPU here; the GPU uses the same code)."""
 import numpy as np
+import pytest
 import torch
 
 from nga2_amd import drift, inputs
@@ -40,3 +41,45 @@ def test_drift_positions_wrap_and_speed():
     assert (p >= 0).all() and (p < 1).all()
     # rms drift of 0.5 cell per step
     assert abs(np.sqrt((v * v).sum(axis=1).mean()) * 256 - 0.5) < 1e-12
+
+
+def _uniform_vf_reference(n, centres, radii, nsub):
+    """Brute force: every cut cell sampled at all nsub^3 leaf midpoints."""
+    from nga2_amd.config import Config, uniform
+
+    h = 1.0 / n
+    xc = (np.arange(n) + 0.5) * h
+    X, Y, Z = np.meshgrid(xc, xc, xc, indexing="ij")
+    off = ((np.arange(nsub) + 0.5) / nsub - 0.5) * h
+    vf = np.zeros((n, n, n))
+    for a in off:
+        for b in off:
+            for c_ in off:
+                ins = np.zeros((n, n, n), dtype=bool)
+                for c, r in zip(centres, radii):
+                    dx = (X - c[0] + 0.5) % 1.0 - 0.5 + a
+                    dy = (Y - c[1] + 0.5) % 1.0 - 0.5 + b
+                    dz = (Z - c[2] + 0.5) % 1.0 - 0.5 + c_
+                    ins |= dx * dx + dy * dy + dz * dz < r * r
+                vf += ins
+    return vf / nsub ** 3
+
+
+@pytest.mark.parametrize("nsub", [8, 16])
+def test_octree_vf_equals_uniform_leaf_sampling(nsub):
+    """The recursive refinement of cut cells (4 levels for nsub = 16, the
+    falling_drop setting amr_ref_lvl = 4) gives the same VF as sampling all
+    nsub^3 leaf midpoints of every cell, including overlapping spheres and
+    periodic images."""
+    from nga2_amd.config import Config, uniform
+
+    n = 12
+    x = uniform(n)
+    cfg = Config(x, x, x, no=1)
+    centres = np.array([[0.3, 0.4, 0.5], [0.45, 0.5, 0.5], [0.95, 0.05, 0.5], [0.7, 0.8, 0.1]])
+    radii = np.array([0.2, 0.15, 0.12, 0.09])
+    got = inputs.sphere_vf(cfg, centres, radii, nsub=nsub)
+    ref = _uniform_vf_reference(n, centres, radii, nsub)
+    assert np.max(np.abs(got - ref)) <= 0.5 / nsub ** 3
+    # and the VF sums to the spheres' volume to the resolution of the leaves
+    assert abs(got.mean() - ref.mean()) < 1e-12

@@ -35,6 +35,27 @@ def _box(nx, ny, nz, per, walls=True):
     return inputs.Problem(cfg, opr, rhs, f"box{nx}x{ny}x{nz}")
 
 
+def _masked_box(nx, ny, nz, per, block):
+    """A box with a solid block (VF = 0): its cells get opr(1) = 0 and become
+    identity rows with rhs = sol = 0 (hypre_str_class.f90:174-175,339-345);
+    the fluid cells around it see homogeneous Neumann faces."""
+    cfg = Config(uniform(nx), uniform(ny), uniform(nz), no=1, xper=per[0], yper=per[1], zper=per[2])
+    cfg.VF[:] = 0.0
+    cfg.VF[cfg.interior] = 1.0
+    (i0, i1), (j0, j1), (k0, k1) = block
+    cfg.VF[1 + i0:1 + i1, 1 + j0:1 + j1, 1 + k0:1 + k1] = 0.0
+    cfg.sync(cfg.VF)
+    cfg.calc_fluid_vol()
+    opr = incomp.laplacian(incomp.Metrics(cfg))
+    act = opr[0][cfg.interior] != 0.0
+    rng = np.random.default_rng(nx * 7 + ny)
+    vals = np.where(act, rng.standard_normal((nx, ny, nz)), 0.0)
+    vals[act] -= vals[act].mean()
+    rhs = cfg.field()
+    rhs[cfg.interior] = vals
+    return inputs.Problem(cfg, opr, rhs, f"masked{nx}x{ny}x{nz}")
+
+
 CASES = {
     "periodic16": lambda: inputs.periodic_gaussian(n=16),
     "channel16": lambda: inputs.channel_spectral(n=16),
@@ -47,6 +68,10 @@ CASES = {
     # (one x-row per wavefront), with walls in y, and in x (row ends)
     "rows256x16x256": lambda: _box(256, 16, 256, (True, False, True)),
     "rows128x64x128": lambda: _box(128, 64, 128, (False, True, True)),
+    # identity rows (opr(1) == 0): a solid block inside a box with walls in
+    # y, and solid spheres immersed in a periodic cube
+    "masked_block16x11x8": lambda: _masked_box(16, 11, 8, (True, False, True), ((5, 9), (3, 7), (2, 5))),
+    "masked_spheres16": lambda: inputs.masked_spheres(n=16, nsph=3, rmin=0.15, rmax=0.3),
 }
 
 

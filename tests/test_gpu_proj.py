@@ -138,3 +138,86 @@ def test_device_operator_feeds_the_solver():
     assert sols[0][0] == sols[1][0]
     assert np.array_equal(sols[0][1], sols[1][1])
     dp.destroy()
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_step_pieces_match_host(idx):
+    """rhs scaling, velocity correction, the two-phase gradient, the CSF jump:
+    bit for bit against the host restatement (same expression order, no FMA)."""
+    name, cfg, rho_faces = _cases()[idx]
+    print(f"\n[step pieces {name}]", flush=True)
+    met = incomp.Metrics(cfg)
+    dp = DeviceProjection(met)
+    shape = cfg.shape
+    No = int(np.prod(shape))
+    rng = np.random.default_rng(100 + idx)
+    if rho_faces is None:
+        rho_faces = tuple(rng.uniform(1.0, 1000.0, shape) for _ in range(3))
+    d_rf = tuple(_dev(r) for r in rho_faces)
+    div = rng.standard_normal(shape)
+    for rho, dt in ((1.0, 0.013), (998.2, 2.5e-3)):
+        d_rhs = torch.full((No,), 7.0, dtype=torch.float64, device=DEV)
+        dp.rhs(_dev(div), rho, dt, d_rhs)
+        assert np.array_equal(_host(d_rhs, shape), incomp.rhs_scaled(cfg, div, rho, dt))
+    grads = [rng.standard_normal(shape) for _ in range(3)]
+    vel = [rng.standard_normal(shape) for _ in range(3)]
+    P = rng.standard_normal(shape)
+    sol = rng.standard_normal(shape)
+    for faces in (None, rho_faces):
+        dvel = [_dev(v) for v in vel]
+        dP = _dev(P)
+        dp.correct(0.01, 1.3, [_dev(g) for g in grads], dvel, rho_faces=None if faces is None else d_rf, P=dP,
+                   sol=_dev(sol))
+        ref, Pn = incomp.correct_velocity(0.01, 1.3, grads, vel, rho_faces=faces, P=P, sol=sol)
+        for ax in range(3):
+            assert np.array_equal(_host(dvel[ax], shape), ref[ax]), ax
+        assert np.array_equal(_host(dP, shape), Pn)
+    dPj = [rng.standard_normal(shape) for _ in range(3)]
+    outs = [torch.full((No,), 7.0, dtype=torch.float64, device=DEV) for _ in range(3)]
+    dp.pgrad_jump(_dev(P), [_dev(a) for a in dPj], outs)
+    refs = incomp.get_pgrad_jump(met, P, dPj)
+    for ax in range(3):
+        assert np.array_equal(_host(outs[ax], shape), refs[ax]), ax
+    # CSF: an SD that vanishes on part of the cells (mycurv = 0 branch)
+    VF = rng.uniform(0, 1, shape)
+    SD = np.where(rng.uniform(0, 1, shape) < 0.4, 0.0, rng.uniform(0.1, 3.0, shape))
+    curv = rng.standard_normal(shape) * 5.0
+    Pj0 = [rng.standard_normal(shape) for _ in range(3)]
+    d_Pj = [_dev(a) for a in Pj0]
+    d_D = [torch.full((No,), 7.0, dtype=torch.float64, device=DEV) for _ in range(3)]
+    d_div = _dev(div)
+    dp.csf_jump(0.072, 1e-3, _dev(SD), _dev(curv), _dev(VF), d_rf, d_Pj, d_D, d_div)
+    Pj, D, dn = incomp.csf_jump(met, 0.072, 1e-3, SD, curv, VF, rho_faces, Pj0, div)
+    for ax in range(3):
+        assert np.array_equal(_host(d_Pj[ax], shape), Pj[ax]), ax
+        assert np.array_equal(_host(d_D[ax], shape), D[ax]), ax
+    assert np.array_equal(_host(d_div, shape), dn)
+    dp.destroy()
+
+
+def test_correct_mfr_matches_host():
+    """correct_mfr through an inflow/outflow pair: the mismatch to 1e-13 of the
+    host's sequential sum, the corrected velocities to 1e-13, and a second
+    call finds no mismatch left."""
+    from test_projection_host import _duct_bconds
+
+    print("\n[correct_mfr]", flush=True)
+    cfg = Config(uniform(20), tanh_stretched(12, 1.0, 2.0), uniform(9), no=2, xper=False, yper=False, zper=True)
+    met = incomp.Metrics(cfg)
+    dp = DeviceProjection(met)
+    bcs = _duct_bconds(cfg)
+    dp.set_bconds(bcs)
+    rng = np.random.default_rng(21)
+    vel = [rng.standard_normal(cfg.shape) for _ in range(3)]
+    for rho in (1.0, 3.7):
+        dvel = [_dev(v) for v in vel]
+        err = dp.correct_mfr(rho, dvel)
+        ref, ref_err = incomp.correct_mfr(cfg, bcs, rho, vel)
+        assert abs(err - ref_err) <= 1e-13 * max(1.0, abs(ref_err)), (err, ref_err)
+        for ax in range(3):
+            assert np.abs(_host(dvel[ax], cfg.shape) - ref[ax]).max() <= 1e-13, ax
+        err2 = dp.correct_mfr(rho, dvel)
+        assert abs(err2) <= 1e-12 * abs(ref_err)
+    dp.set_bconds([])
+    assert dp.correct_mfr(1.0, [_dev(v) for v in vel]) == 0.0
+    dp.destroy()

@@ -226,3 +226,78 @@ def test_full_size_256(which):
         res[prec] = x
         ps.destroy()
     assert rel_err_mean_free(res[0], res[MIXED]) <= 1e-8
+
+
+def test_config1_periodic_gaussian_64():
+    """The 64^3 triply periodic Gaussian case (divergence of seeded N(0,1)
+    face velocities, rho = dt = 1, examples/hit/src/simulation.f90:108-116):
+    FP64 and mixed solves against the FP64 oracle and the FFT closed form of
+    the uniform 7-point operator (fourier3d_class.f90:1103-1196), FP64
+    iteration count within one of the oracle's."""
+    from conftest import closed_form_periodic
+
+    print("\n[config 1, 64^3]", flush=True)
+    P = inputs.periodic_gaussian(n=64)
+    cfg = P.cfg
+    b = interior(P.rhs, cfg)
+    o = Oracle.from_cfg(cfg, P.opr)
+    xr, _ = o.solve(b, rcvg=1e-13, maxit=500)
+    _, st10 = o.solve(b, rcvg=1e-10, maxit=500)
+    o.close()
+    xc = closed_form_periodic(b, 64, 1.0 / 64)
+    assert rel_err_mean_free(xr, xc) <= 1e-10
+    for prec in (0, MIXED):
+        ps = _new(P, precision=prec)
+        ps.init()
+        ps.setup()
+        ps.rhs[:] = P.rhs
+        ps.solve()
+        st = ps.stats()
+        x = interior(ps.sol, cfg)
+        e_o, e_c = rel_err_mean_free(x, xr), rel_err_mean_free(x, xc)
+        print(f"[config 1 prec={prec}] it={ps.it} (oracle {st10['it']}) true={st['true_rerr']:.2e} "
+              f"err oracle {e_o:.2e} closed form {e_c:.2e}", flush=True)
+        assert st["true_rerr"] <= 1e-10 and st["status"] == 0
+        assert e_o <= 1e-8 and e_c <= 1e-8
+        if prec == 0:
+            assert abs(ps.it - st10["it"]) <= 1
+        ps.destroy()
+
+
+@pytest.mark.parametrize("method", [PCG_PFMG, _lib.GMRES_PFMG])
+@pytest.mark.parametrize("nfail", [1, 3, 12, 30])
+def test_setup_after_failed_allocation(method, nfail):
+    """A setup whose nfail-th device allocation fails returns NGAP_ERR_ALLOC;
+    the next setup on the same handle allocates again and the solve converges.
+    The failure is injected on the host (ngapt_fail_alloc) before any kernel
+    of that setup reads the missing buffer."""
+    import ctypes as C
+
+    print(f"\n[failed allocation] method={method} nfail={nfail}", flush=True)
+    P = inputs.periodic_gaussian(n=16)
+    ps = NgapStr(P.cfg, "Pressure", method, 7)
+    ps.stc[:] = STC7
+    ps.maxit, ps.rcvg = 200, 1e-10
+    ps.opr[:] = P.opr
+    ps.init()
+    ps.opts["kdim"] = 5
+    ps.setup()
+    # a changed kdim reallocates the GMRES bases; a fresh level set for PCG
+    ps.opts["kdim"] = 7
+    _lib.check("ngapt_fail_alloc", _lib.lib().ngapt_fail_alloc(ps._h, nfail))
+    try:
+        ps.setup()
+        failed = False
+    except NgapError as e:
+        assert e.code == _lib.ERR_ALLOC
+        failed = True
+    _lib.check("ngapt_fail_alloc", _lib.lib().ngapt_fail_alloc(ps._h, 0))
+    ps.setup_done = False
+    ps.setup()
+    ps.rhs[:] = P.rhs
+    ps.sol[:] = 0.0
+    ps.solve()
+    st = ps.stats()
+    print(f"[failed allocation] injected failure hit: {failed}; it={ps.it} true={st['true_rerr']:.2e}", flush=True)
+    assert st["true_rerr"] <= 1e-10
+    ps.destroy()
