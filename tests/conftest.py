@@ -89,3 +89,42 @@ def rel_err_mean_free(x, xr):
     x = x - x.mean()
     xr = xr - xr.mean()
     return np.linalg.norm(x - xr) / np.linalg.norm(xr)
+
+
+def rel_err_nullspace(x, xr, cfg, opr):
+    """Relative L2 difference after removing, from both solutions, their mean
+    over each connected region of participating cells (opr(1) != 0, linked by
+    nonzero couplings, periodic). The singular system fixes the solution only
+    up to one constant per region; with a single region and no identity rows
+    this is rel_err_mean_free."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import connected_components
+
+    from oracle.orc import interior_opr
+
+    op = interior_opr(opr, cfg)  # [k, j, i, stencil]
+    act = (op[..., 0] != 0.0).ravel()
+    N = cfg.nx * cfg.ny * cfg.nz
+    idx = np.arange(N).reshape(cfg.nz, cfg.ny, cfg.nx)
+    rows, cols = [], []
+    for s, d in ((1, 0), (3, 1), (5, 2)):
+        ax = 2 - d  # array axis of direction d
+        cpl = (op[..., s] != 0.0)
+        if not cfg.periodic[d]:
+            last = [slice(None)] * 3
+            last[ax] = -1
+            cpl[tuple(last)] = False
+        rows.append(idx[cpl])
+        cols.append(np.roll(idx, -1, axis=ax)[cpl])
+    G = sp.coo_matrix((np.ones(sum(len(r) for r in rows)), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(N, N))
+    _, lab = connected_components(G, directed=False)
+    x = np.asarray(x, dtype=np.float64).ravel().copy()
+    xr = np.asarray(xr, dtype=np.float64).ravel().copy()
+    for c in np.unique(lab[act]):
+        m = act & (lab == c)
+        x[m] -= x[m].mean()
+        xr[m] -= xr[m].mean()
+    x[~act] = 0.0
+    xr[~act] = 0.0
+    return np.linalg.norm(x - xr) / np.linalg.norm(xr)
