@@ -133,6 +133,26 @@ int ngap_proj_set_bconds(ngap_proj *p, int nbc, const ngap_bcond *bc, const doub
  * gets the mismatch. Without boundary conditions it does nothing. */
 int ngap_proj_correct_mfr(ngap_proj *p, double rho, double *u, double *v, double *w, double *mfr_error);
 
+/* ngap_proj_shift_p without a handle and without allocating: no counterpart
+ * in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight time-step loops, harnesses that forbid
+ * hipMalloc). vol and vf are the device arrays cfg%vol and cfg%VF, pres the
+ * device pressure (all ghosted, layout above). It removes
+ * sum(vol VF P) / sum(vol VF) over interior cells from the interior cells
+ * with VF > 0 and fills the ghosts, with the same sums and the same result
+ * bits as ngap_proj_create + ngap_proj_shift_p (incomp_class.f90:1720-1740,
+ * config_class.f90:176-214). d_mean (device, may be NULL) gets the removed
+ * value; a fluid volume that is not positive leaves pres unchanged and the
+ * mean NaN. The partial sums live in a fixed 16.4 KB device array of the
+ * library (one per device, held until the process ends), whatever the grid
+ * size; grids of 2^40/7 cells or more are refused (NGAP_ERR_ARG). stream:
+ * a hipStream_t of the library's HIP runtime on which the work is enqueued
+ * (the call then returns without waiting), or NULL for the default stream,
+ * which the call synchronizes before returning. Calls on one device share
+ * the array: calls that overlap in time on one device race. */
+int ngap_proj_shift_p_noalloc(const ngap_grid *g, const double *vol, const double *vf, double *pres, double *d_mean,
+                              void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "ngapproj.h"
@@ -728,3 +729,83 @@ int ngap_proj_correct_mfr(ngap_proj *p, double rho, double *u, double *v, double
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- mean removal without allocation
+// ngap_proj_shift_p_noalloc keeps its partial sums in a file-scope device
+// array instead of memory of an ngap_proj handle: 1024 partials of the fluid
+// volume, 1024 of the weighted sum of P, then the two totals (16.4 KB). The
+// partial count equals that of ngap_proj_create / ngap_proj_shift_p, so the
+// sums, and the result, are the same bits. Every slot is written before it
+// is read within one call, so nothing needs clearing between calls; calls
+// that overlap on one device share the array and race.
+namespace {
+constexpr int NOALLOC_PARTS = 1024;
+constexpr int SCRATCH_DOUBLES = 2 * NOALLOC_PARTS + 2;
+}  // namespace
+
+static __device__ __align__(256) double scratch[SCRATCH_DOUBLES];
+
+namespace {
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[MAX_DEVICES] = {};
+
+// device address of `scratch` on the current device, looked up once per device
+// (each device holds its own copy of the array at its own address)
+int scratch_ptr(double **out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return NGAP_ERR_HIP;
+    if (dev < 0 || dev >= MAX_DEVICES) return NGAP_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) return NGAP_ERR_HIP;
+        g_scratch[dev] = (double *)ptr;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
+// P -= tot[1] / tot[0] on interior cells with VF > 0, tot[0] the fluid
+// volume; mean (optional) = that value. A fluid volume that is not positive
+// leaves P as it is and the mean NaN (ngap_proj_create refuses such a grid).
+__global__ void __launch_bounds__(PT) k_shift_dev(Box b, const double *__restrict__ vf,
+                                                  const double *__restrict__ tot, double *__restrict__ pres,
+                                                  double *__restrict__ mean) {
+    const double fvol = tot[0];
+    const bool ok = fvol > 0.0;
+    const double m = ok ? tot[1] / fvol : NAN;
+    if (mean && blockIdx.x == 0 && threadIdx.x == 0) mean[0] = m;
+    if (!ok) return;
+    for (long long q = blockIdx.x * (long long)PT + threadIdx.x; q < b.No; q += (long long)gridDim.x * PT) {
+        int i, j, k;
+        split(b, q, i, j, k);
+        if (interior(b, i, j, k) && vf[q] > 0.0) pres[q] = pres[q] - m;
+    }
+}
+
+}  // namespace
+
+extern "C" int ngap_proj_shift_p_noalloc(const ngap_grid *g, const double *vol, const double *vf, double *pres,
+                                         double *d_mean, void *stream) {
+    if (!g || !vol || !vf || !pres) return NGAP_ERR_ARG;
+    Box b{};
+    int rc = box_of(g, b);
+    if (rc) return rc;
+    double *sp = nullptr;
+    if ((rc = scratch_ptr(&sp))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    double *tot = sp + 2 * NOALLOC_PARTS;
+    hipLaunchKernelGGL(k_integrate, dim3(NOALLOC_PARTS), dim3(PT), 0, st, b, vol, vf, (const double *)nullptr, sp);
+    hipLaunchKernelGGL(k_integrate, dim3(NOALLOC_PARTS), dim3(PT), 0, st, b, vol, vf, (const double *)pres,
+                       sp + NOALLOC_PARTS);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(PT), 0, st, (const double *)sp, NOALLOC_PARTS, tot);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(PT), 0, st, (const double *)(sp + NOALLOC_PARTS), NOALLOC_PARTS, tot + 1);
+    hipLaunchKernelGGL(k_shift_dev, dim3(blocks_for(b.No)), dim3(PT), 0, st, b, vf, (const double *)tot, pres, d_mean);
+    sync_box(b, st, pres);
+    if (hipGetLastError() != hipSuccess) return NGAP_ERR_HIP;
+    if (!stream && hipStreamSynchronize(nullptr) != hipSuccess) return NGAP_ERR_HIP;
+    return 0;
+}

@@ -42,7 +42,7 @@ def _bind(L):
 
 EXPORTS = ["ngap_proj_create", "ngap_proj_destroy", "ngap_proj_laplacian", "ngap_proj_div", "ngap_proj_pgrad",
            "ngap_proj_shift_p", "ngap_proj_sync", "ngap_proj_rhs", "ngap_proj_correct", "ngap_proj_pgrad_jump",
-           "ngap_proj_csf_jump", "ngap_proj_set_bconds", "ngap_proj_correct_mfr"]
+           "ngap_proj_csf_jump", "ngap_proj_set_bconds", "ngap_proj_correct_mfr", "ngap_proj_shift_p_noalloc"]
 
 
 class Bcond(C.Structure):
