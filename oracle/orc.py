@@ -50,6 +50,8 @@ def lib():
         L.orc_solve.argtypes = [C.c_void_p, _dp, _dp, C.c_int, C.c_double, C.c_int, _ip, _dp, _dp]
         L.orc_pcg_dense.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, _ip, _dp, _dp]
         L.orc_num_threads.restype = C.c_int
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = None
         _LIB = L
     return _LIB
 
@@ -212,3 +214,8 @@ def pcg_dense(A, b, Minv=None, maxit=1000, rcvg=1e-12, flexible=False, x0=None):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_threads(n):
+    """Threads of the oracle's following OpenMP regions."""
+    lib().orc_set_threads(int(n))

@@ -793,3 +793,12 @@ int orc_num_threads(void) {
     return 1;
 #endif
 }
+
+/* threads of the following parallel regions (the CPU baseline's 1-core run) */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
