@@ -12,9 +12,10 @@ Outside the timed region the line also carries:
 - the learned-guess sequence (128^3 periodic, 200 correlated right-hand
   sides): train on steps 0-149, then iterations and wall time on steps
   150-199 against the previous-step guess (nga2_amd.learned);
-- the drifting-spheres two-phase sequence (256^3, 200 spheres): operator
-  assembled on the device every step, setup + solve of the mixed solve from
-  the learned prediction against plain FP64 (nga2_amd.drift);
+- the drifting-spheres two-phase sequence (512^3, 200 spheres, 100 steps):
+  operator and a new right-hand side built on the device every step, setup +
+  solve per step of plain FP64, mixed, and mixed with the U-Net guess and
+  flexible-PCG stage (nga2_amd.drift), one step against the FP64 CPU oracle;
 - fourier3d, the FFT direct solve, at 256^3 triply periodic (nga2_amd.fourier),
   against the PCG solve of the same system;
 - gmres_pfmg on a nonsymmetric implicit-solve system on the 256^3 channel
@@ -26,7 +27,7 @@ Outside the timed region the line also carries:
   python bench.py [--steps K] [--warmup W] [--n 256] [--workload channel_spectral]
                   [--no-cpu] [--cpu-threads T] [--roofline-reps R]
                   [--no-two-phase] [--no-sequence] [--seq-n 128] [--seq-m 8] [--seq-epochs 6]
-                  [--no-drift] [--drift-n 256] [--drift-steps 24] [--no-fourier] [--no-implicit]
+                  [--no-drift] [--drift-n 512] [--drift-steps 100] [--drift-train 12] [--no-fourier] [--no-implicit]
                   [--no-uns] [--uns-n 128]
 """
 from __future__ import annotations
@@ -60,6 +61,8 @@ def parse():
     ap.add_argument("--rcvg", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu-1core", action="store_true", help="skip the one-core CPU oracle solve")
+    ap.add_argument("--no-host-path", action="store_true", help="skip the host-array (ngap_solve) leg")
     ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-two-phase", action="store_true", help="skip the 256^3 two-phase spheres leg")
@@ -69,14 +72,42 @@ def parse():
     ap.add_argument("--no-implicit", action="store_true", help="skip the gmres_pfmg implicit-solve leg")
     ap.add_argument("--no-uns", action="store_true", help="skip the hypre_uns (pcg_amg) masked-solids leg")
     ap.add_argument("--uns-n", type=int, default=128)
-    ap.add_argument("--drift-n", type=int, default=256)
-    ap.add_argument("--drift-steps", type=int, default=24)
+    ap.add_argument("--drift-n", type=int, default=512)
+    ap.add_argument("--drift-steps", type=int, default=100)
+    ap.add_argument("--drift-train", type=int, default=12, help="drift steps whose solves train the U-Net")
+    ap.add_argument("--no-drift-unet", action="store_true", help="drift leg without the U-Net path")
     ap.add_argument("--seq-n", type=int, default=128)
     ap.add_argument("--seq-m", type=int, default=8)
     ap.add_argument("--seq-epochs", type=int, default=6)
     ap.add_argument("--kernel-only", choices=["rbgs", "spmv", "restrict", "prolong"], default=None,
                     help="time one fine-level kernel (for rocprofv3 summaries and counters) and exit")
     return ap.parse_args()
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo) and socket/core counts."""
+    import subprocess
+
+    out = {}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                out[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if "Model name" not in out:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        out["Model name"] = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    out["os.cpu_count"] = os.cpu_count()
+    return out
 
 
 def profiled_traffic(name):
@@ -124,6 +155,48 @@ def kernel_only(args):
                       "reps": args.roofline_reps}), flush=True)
     ds.destroy()
     return 0
+
+
+def host_path_leg(args, P, device_ms):
+    """The solve through the host entry point (ngap_solve), as the Fortran
+    drop-in calls it: rhs and the guess in pageable host arrays, sol copied
+    back (hypre_str_solve, src/solver/hypre_str_class.f90:336-377). Same
+    workload and precision as the headline; 1 warm-up, median of 5."""
+    from nga2_amd._lib import MIXED, PCG_PFMG
+    from nga2_amd.linsol import NgapStr
+
+    cfg = P.cfg
+    ps = NgapStr(cfg, "Pressure", PCG_PFMG, 7)
+    ps.stc[:] = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]])
+    ps.maxit, ps.rcvg, ps.precision = 500, args.rcvg, MIXED
+    ps.opr[:] = P.opr
+    ps.init()
+    ps.setup()
+    walls, copies, its = [], [], []
+    for rep in range(6):
+        ps.rhs[:] = P.rhs
+        ps.sol[:] = 0.0
+        t0 = time.perf_counter()
+        ps.solve()
+        w = (time.perf_counter() - t0) * 1e3
+        st = ps.stats()
+        if rep:
+            walls.append(w)
+            copies.append(st["copy_ms"])
+            its.append(ps.it)
+    ps.destroy()
+    wall = float(np.median(walls))
+    gb = 3 * cfg.nxo * cfg.nyo * cfg.nzo * 8 / 1e9
+    res = {"value": round(1e3 / wall, 4), "unit": "solves/s", "ms_per_solve": round(wall, 3),
+           "copy_ms": round(float(np.median(copies)), 3), "iters": its[0],
+           "ratio_to_device_path": round(wall / device_ms, 3),
+           "host_bytes_gb": round(gb, 3),
+           "note": "ngap_solve with pageable host arrays (rhs, guess in; sol out, ghosted), median of 5; "
+                   "copies staged through pinned chunks on 8 host threads, an all-zero guess chunk cleared "
+                   "on the device instead of copied"}
+    log(f"[bench] host path: {res['value']:.2f} solves/s, copy {res['copy_ms']:.1f} ms, "
+        f"{res['ratio_to_device_path']:.2f}x the device path")
+    return res
 
 
 def two_phase_leg(args, dev, solves=3):
@@ -438,6 +511,10 @@ def main():
         ds_fp64, fp64 = run_mode(FP64, max(1, args.steps // 2), 1)
         log(f"[bench] fp64: {len(fp64['its']) / fp64['elapsed']:.2f} solves/s, it={fp64['its'][0]}")
 
+    host_res = None
+    if not args.no_host_path:
+        host_res = host_path_leg(args, P, 1e3 * mixed["elapsed"] / args.steps)
+
     # roofline of the dominant kernel (fine-level FP32 red/black GS colour pass)
     hk = ds_mixed.hooks()
     rb_ms, rb_bytes = hk.time_kernel(_lib.K_RBGS, prec=1, reps=args.roofline_reps)
@@ -470,21 +547,38 @@ def main():
     cpu = None
     err_mixed = err_fp64 = None
     if not args.no_cpu:
-        from oracle.orc import Oracle, interior, num_threads
+        from oracle.orc import Oracle, interior, num_threads, set_threads
 
         if args.cpu_threads > 0:
-            os.environ["OMP_NUM_THREADS"] = str(args.cpu_threads)
-        log("[bench] CPU oracle (FP64 PCG-PFMG, OpenMP)")
+            set_threads(args.cpu_threads)
+        log("[bench] CPU oracle (FP64 PCG-PFMG, OpenMP): 1 warm-up + median of 5, then one solve on 1 core")
         ts = time.perf_counter()
         o = Oracle.from_cfg(cfg, P.opr)
         osetup = time.perf_counter() - ts
         b = interior(P.rhs, cfg)
-        ts = time.perf_counter()
-        xr, st_r = o.solve(b, rcvg=args.rcvg, maxit=500)
-        osolve = time.perf_counter() - ts
-        cpu = {"value": round(1.0 / osolve, 4), "unit": "solves/s", "cores": num_threads(), "kind": "port",
-               "sample": f"one full solve of {P.name} to {args.rcvg:g} (setup {osetup:.1f}s excluded)",
-               "iters": st_r["it"], "solve_s": round(osolve, 3)}
+        threads = num_threads()
+        times = []
+        for rep in range(6):
+            ts = time.perf_counter()
+            xr, st_r = o.solve(b, rcvg=args.rcvg, maxit=500)
+            if rep:
+                times.append(time.perf_counter() - ts)
+        osolve = float(np.median(times))
+        one = None
+        if not args.no_cpu_1core:
+            set_threads(1)
+            ts = time.perf_counter()
+            o.solve(b, rcvg=args.rcvg, maxit=500)
+            one = time.perf_counter() - ts
+            set_threads(threads)
+        cpu = {"value": round(1.0 / osolve, 4), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"{P.name}, full solve to {args.rcvg:g}: 1 warm-up then the median of 5 "
+                         f"(setup {osetup:.1f}s excluded)",
+               "cpu_model": cpu_model(), "iters": st_r["it"], "solve_s_median": round(osolve, 3),
+               "solve_s_runs": [round(t, 3) for t in times]}
+        if one is not None:
+            cpu["one_core"] = {"value": round(1.0 / one, 4), "solve_s": round(one, 3), "cores": 1,
+                               "sample": "one solve after the threaded runs"}
         xr -= xr.mean()
 
         def err(sol):
@@ -527,12 +621,38 @@ def main():
     if not args.no_drift:
         from nga2_amd import drift
 
-        log(f"[bench] drifting-spheres sequence ({args.drift_n}^3, 200 spheres, density ratio 1000)")
-        drift_res = drift.run(n=args.drift_n, steps=args.drift_steps, fit_steps=16, m=6, drift_cells=0.5,
-                              oracle_steps=() if args.no_cpu else (args.drift_steps - 1,), log=log)
+        checks = {}
+
+        def drift_check(step, seq, x64, xmx):
+            # the FP64 CPU oracle on this step's operator and right-hand side
+            from oracle.orc import Oracle, interior
+
+            c = seq.cfg
+            opr_h = seq.opr.cpu().numpy().reshape((7,) + c.shape, order="F")
+            b = interior(seq.rhs.cpu().numpy().reshape(c.shape, order="F"), c)
+            ts = time.perf_counter()
+            o = Oracle.from_cfg(c, opr_h)
+            xr, st_r = o.solve(b, rcvg=1e-12, maxit=500)
+            o.close()
+            xr = xr - xr.mean()
+            nr = np.linalg.norm(xr)
+            checks[step] = {"err_fp64_vs_oracle": float(np.linalg.norm(x64.cpu().numpy() - xr) / nr),
+                            "err_mixed_vs_oracle": float(np.linalg.norm(xmx.cpu().numpy() - xr) / nr),
+                            "oracle_it": st_r["it"], "oracle_s": round(time.perf_counter() - ts, 1)}
+            log(f"[bench] drift step {step} vs FP64 oracle: {checks[step]}")
+
+        log(f"[bench] drifting-spheres sequence ({args.drift_n}^3, 200 spheres, density ratio 1000, "
+            f"{args.drift_steps} steps)")
+        drift_res = drift.run(n=args.drift_n, steps=args.drift_steps, train_steps=args.drift_train,
+                              unet=not args.no_drift_unet, check=drift_check,
+                              check_steps=() if args.no_cpu else (args.drift_steps - 1,), log=log)
+        if checks:
+            drift_res["oracle_check"] = checks
+            drift_res["max_err_mixed_vs_fp64_oracle"] = max(v["err_mixed_vs_oracle"] for v in checks.values())
         drift_res["workload"] = (f"{args.drift_n}^3 periodic, 200 spheres r 0.02-0.08 drifting 0.5 cell/step (rms), "
-                                 "density ratio 1000, operator assembled on the device every step, fixed RHS; "
-                                 "setup + solve per step, rcvg 1e-10")
+                                 "density ratio 1000, operator assembled on the device every step, a new "
+                                 "configuration-1 RHS every step; zero guess, setup + solve per step, rcvg 1e-10; "
+                                 f"U-Net trained on crops of steps 0-{args.drift_train - 1}, medians over the rest")
 
     steps = args.steps
     el = mixed["elapsed"]
@@ -555,6 +675,8 @@ def main():
                              "err_vs_fp64_oracle": err_fp64,
                              "ms_per_solve": round(1e3 * fp64["elapsed"] / len(fp64["its"]), 3)}
         out["speedup_mixed_vs_fp64"] = round(out["value"] / out["fp64_plain"]["value"], 3)
+    if host_res:
+        out["host_path"] = host_res
     if two_phase:
         out["two_phase_256"] = two_phase
     if sequence:

@@ -75,9 +75,16 @@ int ngapnet_train_step(ngapnet *m, const float *x, const float *target, float lr
 /* output scale of the correction stage */
 int ngapnet_set_kappa(ngapnet *m, double kappa);
 /* Correction stage with the signature of ngap_stage_fn (ngapsolve.h):
- * e = s * kappa * net(r / s), s = rms(r), r and e device FP64 vectors of n cells.
+ * e = s * kappa * net(d r / s), s = rms(d r), r and e device FP64 vectors of n cells,
+ * d the input scale of ngapnet_set_input_scale (1 when none is set).
  * ctx is the ngapnet object. */
 int ngapnet_stage(void *ctx, const double *r, double *e, long long n, void *stream);
+/* Per-cell input scale d of the stage (device FP64, n cells, x fastest), for
+ * example 1 / diag(A): the network then sees the Jacobi-scaled residual, which
+ * carries the local coefficient (the density of a two-phase operator). The
+ * array is the caller's and must stay valid while the stage is attached;
+ * d == NULL removes the scale. No counterpart in the reference. */
+int ngapnet_set_input_scale(ngapnet *m, const double *d, long long n);
 #ifdef __cplusplus
 }
 #endif

@@ -76,6 +76,13 @@ extern "C" {
 /* ngap_stats.status */
 #define NGAP_STATUS_CONVERGED 0
 #define NGAP_STATUS_MAXIT 1
+/* PCG stopped before maxit: the recursive residual found no new value 10%
+ * below its best for NGAP_STAG_WINDOW iterations in a row, as when the
+ * right-hand side of a singular operator has a component in its null space
+ * (a periodic or all-Neumann box whose rhs does not sum to zero). Like
+ * MAXIT it is not an error: it and rerr/true_rerr hold the last iterate. */
+#define NGAP_STATUS_STAGNATED 2
+#define NGAP_STAG_WINDOW 25
 
 typedef struct ngap_t ngap_t;
 

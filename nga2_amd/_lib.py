@@ -20,7 +20,7 @@ ERR_ARG, ERR_STENCIL, ERR_NOT_INIT, ERR_NOT_SETUP, ERR_METHOD, ERR_UNSUPPORTED =
 ERR_NOT_CIRCULANT = -7
 ERR_HIP, ERR_NONFINITE, ERR_BREAKDOWN, ERR_ALLOC, ERR_COARSE, ERR_STAGE, ERR_SINGULAR = 1, 2, 3, 4, 5, 6, 7
 STAGE_GUESS, STAGE_PRECOND = 1, 2
-STATUS_CONVERGED, STATUS_MAXIT = 0, 1
+STATUS_CONVERGED, STATUS_MAXIT, STATUS_STAGNATED = 0, 1, 2
 
 K_SPMV, K_RBGS, K_RESTRICT, K_PROLONG, K_UPDATE, K_EXACT, K_SWEEP2, K_SWEEP2_ZERO = 0, 1, 2, 3, 4, 5, 6, 7
 

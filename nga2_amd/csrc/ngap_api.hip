@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <type_traits>
 #include <vector>
 
@@ -557,6 +558,20 @@ double now_ms() {
 
 // ====================================================================== handle
 
+// Host staging of the host entry points (ngap_solve). The caller's arrays are
+// pageable. They move through pinned chunks: STG_THREADS host threads each own
+// a contiguous part of the array, two pinned chunks and a HIP stream, and copy
+// chunk by chunk between pageable and pinned memory while the DMA engine moves
+// the other chunk of the pair.
+constexpr int STG_THREADS = 8;
+constexpr size_t STG_CHUNK = (size_t)4 << 20;  // bytes per pinned chunk
+struct Staging {
+    char *pin = nullptr;  // 2 * STG_THREADS chunks
+    hipStream_t st[STG_THREADS] = {};
+    hipEvent_t ev[STG_THREADS][2] = {};
+    bool ready = false;
+};
+
 struct ngap_t {
     ngap_grid g{};
     int nst = 0;
@@ -585,6 +600,7 @@ struct ngap_t {
     int *flags = nullptr;
     double *h_scal = nullptr;  // pinned
     int *h_flags = nullptr;    // pinned, inside the h_scal allocation
+    hipEvent_t ev_scal = nullptr;  // marks the copy of the scalars into h_scal
     double *Ainv = nullptr;
     int nc = 0;
     int coarse_dense = 0, coarse_singular = 0;
@@ -606,6 +622,8 @@ struct ngap_t {
     double *gm = nullptr;    // device scalars: [0,64) dots, [64,128) second-pass dots, 128 ||w||^2, [192,256) y
     double *h_gm = nullptr;  // pinned copy
     int fail_alloc = 0;      // > 0: the fail_alloc-th next device allocation fails (ngapt_fail_alloc)
+    Staging stg;             // pinned staging of ngap_solve, made on its first call
+    long long zero_chunks = 0;  // guess chunks of the last ngap_solve found all zero (not copied)
 };
 
 namespace {
@@ -1319,6 +1337,35 @@ int guess_stage(ngap_t *h, double bn, double *rel) {
     return 0;
 }
 
+// p <- z + beta p with z = M r (preconditioner, plus the stage when `stage`):
+// r.z (and z.q for the flexible beta), beta, rho, then p
+int next_dir(ngap_t *h, bool mixed, bool stage, bool flexible) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    bool z64;
+    int rc = precond_stage(h, mixed, stage, &z64);
+    if (rc) return rc;
+    if (!z64)
+        hipLaunchKernelGGL(k_dot_rz<float>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r, (const float *)h->z32,
+                           flexible ? (const double *)h->q : (const double *)nullptr, h->parts,
+                           h->parts + MAX_PARTS);
+    else
+        hipLaunchKernelGGL(k_dot_rz<double>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r, (const double *)h->z,
+                           flexible ? (const double *)h->q : (const double *)nullptr, h->parts,
+                           h->parts + MAX_PARTS);
+    hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, (const double *)h->parts,
+                       (const double *)(h->parts + MAX_PARTS), nb, h->scal, h->flags, flexible ? 1 : 0);
+    if (!z64)
+        hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, s, N, h->p, (const float *)h->z32,
+                           (const double *)h->scal, 0);
+    else
+        hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, s, N, h->p, (const double *)h->z,
+                           (const double *)h->scal, 0);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 int run_pcg(ngap_t *h) {
     const long long N = h->g0.N;
     const int nb = grid_for(N);
@@ -1359,6 +1406,11 @@ int run_pcg(ngap_t *h) {
     int it = 0;
     bool restart = true, first = true, converged = false;
     double tr = rel;
+    double q = 1.0;  // last per-iteration reduction of rel
+    bool have_q = false;
+    double best = rel;  // lowest rel so far, and iterations since it last fell by 10%
+    int since_best = 0;
+    bool stagnated = false;
     if (mixed) launch_to_f32(s, N, h->r, h->r32);
     while (true) {
         if (rel <= tol) {
@@ -1393,8 +1445,25 @@ int run_pcg(ngap_t *h) {
                            (int)S_RR);
         HIPCHK(hipGetLastError());
         it++;
-        if ((rc = read_scal(h))) return rc;
+        // Read ||r||^2 and the flags without idling the device: unless this
+        // iteration is predicted to be the last, the next direction (V-cycle,
+        // r.z, beta, p) is enqueued behind the copy and runs while the host
+        // waits for it. That work reads r and writes z, p and the beta/rho
+        // scalars only, so when the read then says "converged" or "breakdown"
+        // it is discarded: x and r are as without it, and every path that
+        // continues from there recomputes p from a fresh preconditioned r.
+        const double pred = rel * (have_q ? q : 1.0);
+        const bool spec = pred > 4.0 * tol && it < h->maxit;
+        HIPCHK(hipMemcpyAsync(h->h_scal, h->scal, SCAL_BYTES, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipEventRecord(h->ev_scal, s));
+        if (spec && (rc = next_dir(h, mixed, stage_pc && it < h->stage_iters, flexible))) return rc;
+        HIPCHK(hipEventSynchronize(h->ev_scal));
+        const double rel_prev = rel;
         rel = std::sqrt(h->h_scal[S_RR]) / bn;
+        if (rel_prev > 0.0 && std::isfinite(rel)) {
+            q = std::min(1.0, rel / rel_prev);
+            have_q = true;
+        }
         if (h->h_flags[0]) {
             // preconditioner or step breakdown: x, r untouched by a zero step
             hipLaunchKernelGGL(k_clear_flags, dim3(1), dim3(64), 0, s, h->flags);
@@ -1418,27 +1487,17 @@ int run_pcg(ngap_t *h) {
             continue;
         }
         if (!std::isfinite(rel)) return NGAP_ERR_BREAKDOWN;
+        if (rel < 0.9 * best) {
+            best = rel;
+            since_best = 0;
+        } else if (++since_best >= NGAP_STAG_WINDOW && rel > tol) {
+            stagnated = true;
+            break;
+        }
         if (rel <= tol || it >= h->maxit) continue;
-        bool z64;
-        if ((rc = precond_stage(h, mixed, stage_pc && it < h->stage_iters, &z64))) return rc;
-        if (!z64)
-            hipLaunchKernelGGL(k_dot_rz<float>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r,
-                               (const float *)h->z32, flexible ? (const double *)h->q : (const double *)nullptr,
-                               h->parts, h->parts + MAX_PARTS);
-        else
-            hipLaunchKernelGGL(k_dot_rz<double>, dim3(nb), dim3(NT), 0, s, N, (const double *)h->r,
-                               (const double *)h->z, flexible ? (const double *)h->q : (const double *)nullptr,
-                               h->parts, h->parts + MAX_PARTS);
-        hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, (const double *)h->parts,
-                           (const double *)(h->parts + MAX_PARTS), nb, h->scal, h->flags, flexible ? 1 : 0);
-        if (!z64)
-            hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, s, N, h->p, (const float *)h->z32,
-                               (const double *)h->scal, 0);
-        else
-            hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, s, N, h->p, (const double *)h->z,
-                               (const double *)h->scal, 0);
+        if (!spec && (rc = next_dir(h, mixed, stage_pc && it < h->stage_iters, flexible))) return rc;
     }
-    if (!converged && it >= h->maxit && rel > tol) {
+    if (!converged && (stagnated || it >= h->maxit) && rel > tol) {
         exact_residual(h, h->q, S_RRT);
         HIPCHK(hipGetLastError());
         if ((rc = read_scal(h))) return rc;
@@ -1450,7 +1509,7 @@ int run_pcg(ngap_t *h) {
     h->st.aerr = tr * bn;
     double rho0 = h->h_scal[S_RHO0], rho = h->h_scal[S_RHO];
     h->st.cnorm_ratio = (rho0 > 0.0 && rho >= 0.0) ? std::sqrt(rho / rho0) : 0.0;
-    h->st.status = converged ? NGAP_STATUS_CONVERGED : NGAP_STATUS_MAXIT;
+    h->st.status = converged ? NGAP_STATUS_CONVERGED : (stagnated ? NGAP_STATUS_STAGNATED : NGAP_STATUS_MAXIT);
     h->st.precision = mixed ? NGAP_MIXED : NGAP_FP64;
     return 0;
 }
@@ -1626,6 +1685,132 @@ int run_gmres(ngap_t *h) {
     return 0;
 }
 
+__global__ void __launch_bounds__(256) k_zero_fill(double *__restrict__ p, long long n) {
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) p[i] = 0.0;
+}
+
+void free_staging(Staging &S) {
+    for (int t = 0; t < STG_THREADS; t++) {
+        for (int k = 0; k < 2; k++)
+            if (S.ev[t][k]) (void)hipEventDestroy(S.ev[t][k]);
+        if (S.st[t]) (void)hipStreamDestroy(S.st[t]);
+    }
+    if (S.pin) (void)hipHostFree(S.pin);
+    S = Staging{};
+}
+
+// the pinned chunks, streams and events (once per handle); false: not
+// available, the caller copies directly from pageable memory
+bool make_staging(Staging &S) {
+    if (S.ready) return true;
+    bool ok = hipHostMalloc((void **)&S.pin, 2 * STG_THREADS * STG_CHUNK, hipHostMallocDefault) == hipSuccess;
+    for (int t = 0; ok && t < STG_THREADS; t++) {
+        ok = hipStreamCreateWithFlags(&S.st[t], hipStreamNonBlocking) == hipSuccess;
+        for (int k = 0; ok && k < 2; k++)
+            ok = hipEventCreateWithFlags(&S.ev[t][k], hipEventDisableTiming) == hipSuccess;
+    }
+    if (!ok) {
+        free_staging(S);
+        return false;
+    }
+    S.ready = true;
+    return true;
+}
+
+bool all_zero(const double *p, size_t n) {
+    // +0.0 and -0.0 both count as zero, as the guess's value is what matters
+    for (size_t i = 0; i < n; i++)
+        if (p[i] != 0.0) return false;
+    return true;
+}
+
+// One thread's part of a host -> device copy: bytes from src (pageable) to
+// dst (device) through the thread's two pinned chunks. skip_zero: a chunk
+// that is all zero is cleared on the device by a fill kernel instead of being
+// copied (counted in *zeroed).
+int h2d_part(const Staging &S, int t, int device, char *dst, const char *src, size_t bytes, bool skip_zero,
+             long long *zeroed) {
+    if (hipSetDevice(device) != hipSuccess) return NGAP_ERR_HIP;
+    char *pin[2] = {S.pin + (size_t)(2 * t) * STG_CHUNK, S.pin + (size_t)(2 * t + 1) * STG_CHUNK};
+    int k = 0;
+    for (size_t off = 0; off < bytes; off += STG_CHUNK, k++) {
+        const size_t n = std::min(STG_CHUNK, bytes - off);
+        const int sl = k & 1;
+        if (skip_zero && all_zero((const double *)(src + off), n / sizeof(double))) {
+            const long long m = (long long)(n / sizeof(double));
+            hipLaunchKernelGGL(k_zero_fill, dim3((unsigned)std::min<long long>((m + 255) / 256, 4096)), dim3(256), 0,
+                               S.st[t], (double *)(dst + off), m);
+            (*zeroed)++;
+            continue;
+        }
+        // the chunk's previous DMA must have left the pinned buffer
+        if (k >= 2 && hipEventSynchronize(S.ev[t][sl]) != hipSuccess) return NGAP_ERR_HIP;
+        std::memcpy(pin[sl], src + off, n);
+        if (hipMemcpyAsync(dst + off, pin[sl], n, hipMemcpyHostToDevice, S.st[t]) != hipSuccess ||
+            hipEventRecord(S.ev[t][sl], S.st[t]) != hipSuccess)
+            return NGAP_ERR_HIP;
+    }
+    if (hipGetLastError() != hipSuccess) return NGAP_ERR_HIP;
+    return hipStreamSynchronize(S.st[t]) == hipSuccess ? 0 : NGAP_ERR_HIP;
+}
+
+// One thread's part of a device -> host copy: the DMA of chunk k+1 runs while
+// chunk k is copied from pinned to pageable memory.
+int d2h_part(const Staging &S, int t, int device, char *dst, const char *src, size_t bytes) {
+    if (hipSetDevice(device) != hipSuccess) return NGAP_ERR_HIP;
+    char *pin[2] = {S.pin + (size_t)(2 * t) * STG_CHUNK, S.pin + (size_t)(2 * t + 1) * STG_CHUNK};
+    const size_t nch = (bytes + STG_CHUNK - 1) / STG_CHUNK;
+    auto issue = [&](size_t k) {
+        const size_t off = k * STG_CHUNK, n = std::min(STG_CHUNK, bytes - off);
+        return hipMemcpyAsync(pin[k & 1], src + off, n, hipMemcpyDeviceToHost, S.st[t]) == hipSuccess &&
+               hipEventRecord(S.ev[t][k & 1], S.st[t]) == hipSuccess;
+    };
+    if (nch > 0 && !issue(0)) return NGAP_ERR_HIP;
+    for (size_t k = 0; k < nch; k++) {
+        if (k + 1 < nch && !issue(k + 1)) return NGAP_ERR_HIP;
+        if (hipEventSynchronize(S.ev[t][k & 1]) != hipSuccess) return NGAP_ERR_HIP;
+        const size_t off = k * STG_CHUNK, n = std::min(STG_CHUNK, bytes - off);
+        std::memcpy(dst + off, pin[k & 1], n);
+    }
+    return 0;
+}
+
+// Copy `bytes` (a multiple of 8) between the caller's pageable array and a
+// device array on STG_THREADS threads; parts are contiguous and 8-byte
+// aligned. A thread that cannot be started runs its part on the caller's
+// thread. The device array is complete (H2D) or the host array written (D2H)
+// when this returns 0.
+int staged_copy(ngap_t *h, bool to_device, char *dev, char *host, size_t bytes, bool skip_zero) {
+    Staging &S = h->stg;
+    const size_t words = bytes / sizeof(double);
+    int rcs[STG_THREADS] = {};
+    long long zs[STG_THREADS] = {};
+    auto part = [&](int t) {
+        const size_t w0 = words * (size_t)t / STG_THREADS, w1 = words * (size_t)(t + 1) / STG_THREADS;
+        const size_t o = w0 * sizeof(double), n = (w1 - w0) * sizeof(double);
+        rcs[t] = to_device ? h2d_part(S, t, h->device, dev + o, host + o, n, skip_zero, &zs[t])
+                           : d2h_part(S, t, h->device, host + o, dev + o, n);
+    };
+    std::vector<std::thread> th;
+    std::vector<int> inline_parts;
+    for (int t = 1; t < STG_THREADS; t++) {
+        try {
+            th.emplace_back(part, t);
+        } catch (...) {
+            inline_parts.push_back(t);
+        }
+    }
+    part(0);
+    for (int t : inline_parts) part(t);
+    for (auto &x : th) x.join();
+    h->zero_chunks = 0;
+    for (int t = 0; t < STG_THREADS; t++) {
+        if (rcs[t]) return rcs[t];
+        h->zero_chunks += zs[t];
+    }
+    return hipSetDevice(h->device) == hipSuccess ? 0 : NGAP_ERR_HIP;
+}
+
 int do_solve(ngap_t *h, const double *rhs, double *sol, bool on_device, int *it, double *rerr, double *aerr) {
     if (!h || !rhs || !sol) return NGAP_ERR_ARG;
     if (!h->setup_done) return NGAP_ERR_NOT_SETUP;
@@ -1641,9 +1826,17 @@ int do_solve(ngap_t *h, const double *rhs, double *sol, bool on_device, int *it,
         HIPCHK(hipDeviceSynchronize());
     } else {
         double c0 = now_ms();
-        HIPCHK(hipMemcpyAsync(h->rhs_g, rhs, gb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipMemcpyAsync(h->sol_g, sol, gb, hipMemcpyHostToDevice, s));
-        HIPCHK(hipStreamSynchronize(s));
+        if (make_staging(h->stg)) {
+            // the previous solve's work on the handle's stream may still read rhs_g / sol_g
+            HIPCHK(hipStreamSynchronize(s));
+            int rc = staged_copy(h, true, (char *)h->rhs_g, (char *)rhs, gb, false);
+            if (!rc) rc = staged_copy(h, true, (char *)h->sol_g, (char *)sol, gb, true);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(h->rhs_g, rhs, gb, hipMemcpyHostToDevice, s));
+            HIPCHK(hipMemcpyAsync(h->sol_g, sol, gb, hipMemcpyHostToDevice, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
         tc += now_ms() - c0;
         drhs = h->rhs_g;
         dsol = h->sol_g;
@@ -1670,9 +1863,15 @@ int do_solve(ngap_t *h, const double *rhs, double *sol, bool on_device, int *it,
     hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, (const double *)h->x, dsol);
     HIPCHK(hipGetLastError());
     if (!on_device) {
-        double c0 = now_ms();
-        HIPCHK(hipMemcpyAsync(sol, h->sol_g, gb, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
+        double c0 = now_ms();
+        if (h->stg.ready) {
+            int rc = staged_copy(h, false, (char *)h->sol_g, (char *)sol, gb, false);
+            if (rc) return rc;
+        } else {
+            HIPCHK(hipMemcpyAsync(sol, h->sol_g, gb, hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
+        }
         tc += now_ms() - c0;
     } else {
         HIPCHK(hipStreamSynchronize(s));
@@ -1937,6 +2136,7 @@ int ngap_init(ngap_t *h) {
     if (!h->init_done) {
         if (hipGetDevice(&h->device) != hipSuccess) return NGAP_ERR_HIP;
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return NGAP_ERR_HIP;
+        if (hipEventCreateWithFlags(&h->ev_scal, hipEventDisableTiming) != hipSuccess) return NGAP_ERR_HIP;
     }
     h->init_done = true;
     return 0;
@@ -2014,6 +2214,8 @@ int ngap_destroy(ngap_t *h) {
         free_all(h);
         if (h->h_scal) (void)hipHostFree(h->h_scal);
         if (h->h_gm) (void)hipHostFree(h->h_gm);
+        free_staging(h->stg);
+        if (h->ev_scal) (void)hipEventDestroy(h->ev_scal);
         if (h->stream) (void)hipStreamDestroy(h->stream);
     }
     delete h;

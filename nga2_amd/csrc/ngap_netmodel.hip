@@ -55,10 +55,15 @@ __global__ void __launch_bounds__(256) k_mse(const float *__restrict__ y, const 
     if (threadIdx.x == 0) part[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
-__global__ void __launch_bounds__(256) k_sumsq64(const double *__restrict__ r, double *__restrict__ part, long long n) {
+// sum of squares of d r per block (d == nullptr: d = 1)
+__global__ void __launch_bounds__(256) k_sumsq64(const double *__restrict__ r, const double *__restrict__ d,
+                                                 double *__restrict__ part, long long n) {
     __shared__ double sm[4];
     double acc = 0.0;
-    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) acc += r[i] * r[i];
+    for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256) {
+        const double v = d ? d[i] * r[i] : r[i];
+        acc += v * v;
+    }
     for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = acc;
     __syncthreads();
@@ -84,14 +89,14 @@ __global__ void __launch_bounds__(256) k_fin_mean(const double *__restrict__ par
     }
 }
 
-// x = r / s (FP32), s = rms[1]; zero when s is 0 or not finite
-__global__ void k_scale_in(const double *__restrict__ r, const double *__restrict__ rms, float *__restrict__ x,
-                           long long n) {
+// x = d r / s (FP32), s = rms[1]; zero when s is 0 or not finite (d == nullptr: d = 1)
+__global__ void k_scale_in(const double *__restrict__ r, const double *__restrict__ d,
+                           const double *__restrict__ rms, float *__restrict__ x, long long n) {
     const double s = rms[1];
     const bool ok = s > 0.0 && isfinite(s);
     const double inv = ok ? 1.0 / s : 0.0;
     for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
-        x[i] = (float)(r[i] * inv);
+        x[i] = (float)((d ? d[i] * r[i] : r[i]) * inv);
 }
 
 // e = s kappa y (FP64)
@@ -124,6 +129,7 @@ struct ngapnet {
     __bf16 *gbf;
     double *red = nullptr;  // RED_BLOCKS partials + 2 results
     double kappa = 1.0;
+    const double *dscale = nullptr;  // stage input scale (caller's device array), or none
     int step = 0;
     std::vector<void *> allocs;
 
@@ -396,16 +402,23 @@ int ngapnet_train_step(ngapnet *M, const float *x, const float *target, float lr
     return hip_ok(hipGetLastError());
 }
 
+int ngapnet_set_input_scale(ngapnet *M, const double *d, long long n) {
+    if (!M) return NGAPNET_ERR_ARG;
+    if (d && n != M->V[0]) return NGAPNET_ERR_SHAPE;
+    M->dscale = d;
+    return 0;
+}
+
 int ngapnet_stage(void *ctx, const double *r, double *e, long long n, void *stream) {
     ngapnet *M = (ngapnet *)ctx;
     if (!M || !r || !e) return NGAPNET_ERR_ARG;
     if (n != M->V[0]) return NGAPNET_ERR_SHAPE;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_sumsq64, dim3(RED_BLOCKS), dim3(256), 0, s, r, M->red, n);
+    hipLaunchKernelGGL(k_sumsq64, dim3(RED_BLOCKS), dim3(256), 0, s, r, M->dscale, M->red, n);
     hipLaunchKernelGGL(k_fin_mean, dim3(1), dim3(256), 0, s, (const double *)M->red, RED_BLOCKS, n,
                        M->red + RED_BLOCKS);
-    hipLaunchKernelGGL(k_scale_in, dim3(blocks(n)), dim3(256), 0, s, r, (const double *)(M->red + RED_BLOCKS), M->x,
-                       n);
+    hipLaunchKernelGGL(k_scale_in, dim3(blocks(n)), dim3(256), 0, s, r, M->dscale,
+                       (const double *)(M->red + RED_BLOCKS), M->x, n);
     NCK(forward(M, s));
     hipLaunchKernelGGL(k_scale_out, dim3(blocks(n)), dim3(256), 0, s, (const float *)M->y,
                        (const double *)(M->red + RED_BLOCKS), M->kappa, e, n);

@@ -179,7 +179,8 @@ def fit_kappa(pairs):
 
 def train(net, pairs, kappa, S=None, epochs=8, lr=2e-3, lr_end=1e-4, seed=0, loss="residual", log=None):
     """Adam on the normalised pairs, one field per step, cosine-decayed rate.
-    Returns the per-epoch mean losses.
+    Returns the per-epoch mean losses; the net is left holding the weights
+    of the epoch with the lowest mean loss (not those of the last epoch).
 
     With x = r / s (s = rms(r)) and y = net(x):
     - loss "residual" (needs the solver S) is ||x - kappa A y||^2 / V. This is
@@ -225,6 +226,9 @@ def train(net, pairs, kappa, S=None, epochs=8, lr=2e-3, lr_end=1e-4, seed=0, los
         hist.append(float(np.mean(losses)))
         if log:
             log(f"epoch {ep}: loss {hist[-1]:.4e}")
+        if hist[-1] <= min(hist):
+            best = net.state()
+    net.load_state(best)
     return hist
 
 

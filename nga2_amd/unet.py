@@ -83,6 +83,7 @@ def lib():
         L.ngapnet_train_step.argtypes = [_vp, _vp, _vp, _f, C.POINTER(C.c_double), _vp]
         L.ngapnet_set_kappa.argtypes = [_vp, C.c_double]
         L.ngapnet_stage.argtypes = [_vp, _vp, _vp, _ll, _vp]
+        L.ngapnet_set_input_scale.argtypes = [_vp, _vp, _ll]
         _L = L
     return _L
 
@@ -213,6 +214,21 @@ class UNet:
         _ck(lib().ngapnet_set_kappa(self.h, float(kappa)), "set_kappa")
         self.kappa = float(kappa)
 
+    def set_input_scale(self, d):
+        """Per-cell input scale of the correction stage (FP64 device tensor of
+        nx*ny*nz cells, e.g. 1/diag(A)), or None. The stage then computes
+        e = s kappa net(d r / s) with s = rms(d r). The tensor is kept alive here."""
+        import torch
+
+        if d is None:
+            self._dscale = None
+            _ck(lib().ngapnet_set_input_scale(self.h, None, 0), "set_input_scale")
+            return
+        d = self._vec(d, torch.float64)
+        self._dscale = d
+        _enter()
+        _ck(lib().ngapnet_set_input_scale(self.h, _p(d), self.V), "set_input_scale")
+
     # ------------------------------------------------------------------ compute
     def _vec(self, t, dtype):
         import torch
@@ -295,6 +311,48 @@ class UNet:
         self.set_weights({k: v for k, v in st.items() if k != "kappa"})
         if "kappa" in st:
             self.set_kappa(float(np.asarray(st["kappa"]).reshape(-1)[0]))
+
+    def save(self, path):
+        """Write the weights and kappa to an .npz file together with the
+        SHA-256 of their bytes (see state_digest); load() refuses a file whose
+        digest does not match."""
+        st = self.state()
+        np.savez(path, sha256=np.frombuffer(state_digest(st).encode(), dtype=np.uint8), **st)
+
+    def load(self, path):
+        self.load_state(read_state(path))
+
+
+def state_digest(st):
+    """SHA-256 (hex) over the layer names, shapes and float32 bytes of a
+    state dict, in the library's layer order, then kappa as float64."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name, *_rest in LAYERS:
+        a = np.ascontiguousarray(np.asarray(st[name], dtype=np.float32).reshape(-1))
+        h.update(name.encode())
+        h.update(np.int64(a.size).tobytes())
+        h.update(a.tobytes())
+    h.update(np.asarray(st["kappa"], dtype=np.float64).reshape(-1).tobytes())
+    return h.hexdigest()
+
+
+def read_state(path):
+    """State dict of a file written by UNet.save, after checking its digest
+    (ValueError on a mismatch or a missing digest). Loaded without pickle."""
+    with np.load(path, allow_pickle=False) as z:
+        if "sha256" not in z.files:
+            raise ValueError(f"{path}: no sha256 entry")
+        want = bytes(z["sha256"]).decode()
+        st = {k: z[k].copy() for k in z.files if k != "sha256"}
+    missing = [name for name, *_r in LAYERS if name not in st] + ([] if "kappa" in st else ["kappa"])
+    if missing:
+        raise ValueError(f"{path}: missing {missing}")
+    got = state_digest(st)
+    if got != want:
+        raise ValueError(f"{path}: checksum mismatch (file {want[:16]}..., contents {got[:16]}...)")
+    return st
 
 
 def time_net(n=128, reps=5, device="cuda:0"):

@@ -83,3 +83,62 @@ def test_octree_vf_equals_uniform_leaf_sampling(nsub):
     assert np.max(np.abs(got - ref)) <= 0.5 / nsub ** 3
     # and the VF sums to the spheres' volume to the resolution of the leaves
     assert abs(got.mean() - ref.mean()) < 1e-12
+
+
+def _periodic_operator(n, seed):
+    """A random symmetric periodic 7-point operator with positive diagonal,
+    as (7, n, n, n) [s, k, j, i] in the stencil order (0, +x, -x, +y, -y, +z, -z)."""
+    rng = np.random.default_rng(seed)
+    a = [-rng.uniform(0.001, 1.0, (n, n, n)) for _ in range(3)]  # coupling of a cell to its +d neighbour
+    o = np.zeros((7, n, n, n))
+    for d, ax in enumerate((2, 1, 0)):
+        o[1 + 2 * d] = a[d]
+        o[2 + 2 * d] = np.roll(a[d], 1, axis=ax)
+    o[0] = -o[1:].sum(axis=0) + 0.1
+    return o
+
+
+def test_crop_apply_is_the_principal_submatrix():
+    # crop_apply equals the full periodic stencil on every crop cell whose
+    # neighbours are all inside the crop, and drops the outside couplings
+    n, m, off = 12, 7, (9, 2, 5)
+    o = _periodic_operator(n, 3)
+    x = np.random.default_rng(4).standard_normal((n, n, n))
+    full = o[0] * x
+    for s, (ax, sh) in enumerate(((2, -1), (2, 1), (1, -1), (1, 1), (0, -1), (0, 1))):
+        full += o[1 + s] * np.roll(x, sh, axis=ax)
+    ix = [(np.arange(m) + q) % n for q in off]
+    crop = lambda a: a[np.ix_(ix[2], ix[1], ix[0])]  # noqa: E731
+    oc = torch.from_numpy(np.stack([crop(o[s]) for s in range(7)]))
+    yc = torch.from_numpy(crop(x))
+    got = drift.crop_apply(oc, yc).numpy()
+    assert np.allclose(got[1:-1, 1:-1, 1:-1], crop(full)[1:-1, 1:-1, 1:-1], rtol=1e-13, atol=1e-13)
+    # a field that vanishes inside the crop sees no coupling from outside
+    only_out = x.copy()
+    only_out[np.ix_(ix[2], ix[1], ix[0])] = 0.0
+    assert np.abs(drift.crop_apply(oc, torch.from_numpy(crop(only_out))).numpy()).max() == 0.0
+
+
+def test_crop_loss_gradient_matches_autograd():
+    # the training gradient of the Jacobi-scaled residual loss against torch
+    # autograd (a sign or transpose slip fails it)
+    m = 10
+    o = torch.from_numpy(_periodic_operator(m, 8)).clone()
+    for s, (ax, sl) in enumerate(((2, -1), (2, 0), (1, -1), (1, 0), (0, -1), (0, 0))):
+        idx = [slice(None)] * 3
+        idx[ax] = sl
+        o[1 + s][tuple(idx)] = 0.0  # the crop's outside couplings are absent
+    g = torch.Generator().manual_seed(1)
+    u = torch.randn((m, m, m), generator=g, dtype=torch.float64)
+    y = torch.randn((m, m, m), generator=g, dtype=torch.float64, requires_grad=True)
+    dinv = 1.0 / o[0]
+    mask = torch.zeros((m, m, m), dtype=torch.float64)
+    mask[2:-2, 2:-2, 2:-2] = 1.0
+    kappa = 0.7
+    loss, grad = drift.crop_loss_grad(u, o, dinv, y.detach(), kappa, mask)
+    res = mask * (u - kappa * dinv * drift.crop_apply(o, y))
+    L = (res * res).sum() / mask.sum()
+    L.backward()
+    Lv = float(L.detach())
+    assert abs(loss - Lv) <= 1e-12 * abs(Lv)
+    assert torch.allclose(grad, y.grad, rtol=1e-10, atol=1e-14)

@@ -301,3 +301,37 @@ def test_setup_after_failed_allocation(method, nfail):
     print(f"[failed allocation] injected failure hit: {failed}; it={ps.it} true={st['true_rerr']:.2e}", flush=True)
     assert st["true_rerr"] <= 1e-10
     ps.destroy()
+
+
+@pytest.mark.parametrize("precision", [0, MIXED])
+def test_inconsistent_rhs_stagnates(precision):
+    """A periodic box's operator is singular with the constants as null space,
+    so a right-hand side that does not sum to zero has no solution. The solve
+    stops with NGAP_STATUS_STAGNATED well before maxit instead of running to
+    it, and reports the residual it reached; the same rhs made consistent
+    converges."""
+    print(f"\n[inconsistent rhs, precision {precision}]", flush=True)
+    P = inputs.periodic_gaussian(n=32)
+    cfg = P.cfg
+    ps = _new(P, precision=precision, maxit=400)
+    ps.init()
+    ps.setup()
+    b = P.rhs.copy()
+    b[cfg.interior] += 1e-3 * np.abs(b[cfg.interior]).max()  # a component along the null space
+    ps.rhs[:] = b
+    ps.sol[:] = 0.0
+    ps.solve()
+    st = ps.stats()
+    print(f"  it {ps.it} status {st['status']} rerr {ps.rerr:.3e} true {st['true_rerr']:.3e}", flush=True)
+    assert st["status"] == _lib.STATUS_STAGNATED
+    assert ps.it < 400 and st["true_rerr"] > 1e-10 and np.isfinite(ps.sol).all()
+    # the floor is the null-space share of b: ||mean(b) 1|| / ||b||
+    bi = b[cfg.interior]
+    floor = abs(bi.mean()) * np.sqrt(bi.size) / np.linalg.norm(bi)
+    assert st["true_rerr"] >= 0.99 * floor
+    ps.rhs[:] = P.rhs
+    ps.sol[:] = 0.0
+    ps.solve()
+    st = ps.stats()
+    assert st["status"] == _lib.STATUS_CONVERGED and st["true_rerr"] <= 1e-10
+    ps.destroy()
