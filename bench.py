@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--roofline-reps", type=int, default=20)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--no-two-phase", action="store_true", help="skip the 256^3 two-phase spheres leg")
+    ap.add_argument("--no-two-phase-unet", action="store_true", help="two-phase leg without the U-Net stage")
+    ap.add_argument("--no-device-step", action="store_true", help="skip the device-resident projection step leg")
     ap.add_argument("--no-sequence", action="store_true", help="skip the learned-guess sequence leg")
     ap.add_argument("--no-drift", action="store_true", help="skip the drifting-spheres two-phase sequence leg")
     ap.add_argument("--no-fourier", action="store_true", help="skip the fourier3d (FFT direct solve) leg")
@@ -208,14 +210,14 @@ def two_phase_leg(args, dev, solves=3):
     from nga2_amd.linsol import DeviceSolve
 
     t0 = time.time()
-    P = inputs.spheres_two_phase(n=args.n)
+    P = inputs.spheres_two_phase_config3(n=args.n)
     log(f"[bench] generated {P.name} in {time.time() - t0:.1f}s")
     cfg = P.cfg
     d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
     d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
     d_sol = torch.zeros_like(d_rhs)
-    res = {"workload": f"{P.name}: {cfg.nx}^3 periodic, 50 spheres, density ratio 1000, arithmetic face "
-                       f"densities, rcvg {args.rcvg:g}, zero guess"}
+    res = {"workload": f"{P.name}: {cfg.nx}^3 periodic, 50 spheres r 0.05-0.15, VF by four-level refinement, "
+                       f"density ratio 1000, arithmetic face densities, rcvg {args.rcvg:g}, zero guess"}
     for name, prec in (("mixed", MIXED), ("fp64", FP64)):
         ds = DeviceSolve(cfg)
         ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=prec)
@@ -245,6 +247,163 @@ def two_phase_leg(args, dev, solves=3):
     res["speedup_solve_mixed_vs_fp64"] = round(res["fp64"]["solve_ms"] / res["mixed"]["solve_ms"], 3)
     res["speedup_setup_solve_mixed_vs_fp64"] = round(
         (res["fp64"]["solve_ms"] + res["fp64"]["setup_ms"]) / (res["mixed"]["solve_ms"] + res["mixed"]["setup_ms"]), 3)
+    if not args.no_two_phase_unet:
+        res["unet_mixed"] = two_phase_unet(args, P, d_opr, d_rhs, solves)
+        u = res["unet_mixed"]
+        res["north_star_speedup"] = round(
+            (res["fp64"]["solve_ms"] + res["fp64"]["setup_ms"]) / (u["solve_ms"] + u["setup_ms"]), 3)
+        res["unet_iteration_saving"] = res["mixed"]["iters"] - u["iters"]
+    return res
+
+
+def two_phase_unet(args, P, d_opr, d_rhs, solves):
+    """Mixed precision with the learned stage on the two-phase system: the
+    U-Net (input D^-1 r) is trained on crops of this operator's solves of
+    other configuration-1 right-hand sides (seeds 7001-7004), then gives the
+    initial guess and the first flexible-PCG stage of the timed solves.
+    setup_ms counts the mixed setup and the input-scale update; the training
+    is reported apart (it is paid once per operator, not per solve)."""
+    import torch
+
+    from nga2_amd import _lib, incomp, learned
+    from nga2_amd._lib import MIXED
+    from nga2_amd.linsol import DeviceSolve
+    from nga2_amd.proj import DeviceProjection
+    from nga2_amd.unet import UNet
+
+    cfg = P.cfg
+    ds = DeviceSolve(cfg)
+    ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=MIXED)
+    proj = DeviceProjection(incomp.Metrics(cfg))
+    state, info = learned.train_for_operator(ds, proj, d_opr, cfg, seeds=range(7001, 7005),
+                                             log=lambda m: log("[bench] two-phase " + m))
+    proj.destroy()
+    net = UNet((cfg.nx, cfg.ny, cfg.nz), seed=0)
+    net.load_state(state)
+    dinv = torch.zeros(cfg.nx * cfg.ny * cfg.nz, dtype=torch.float64, device=d_opr.device)
+    d_sol = torch.zeros_like(d_rhs)
+    setup_ms, solve_ms, its, st = [], [], [], None
+    for rep in range(solves + 1):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=MIXED)
+        learned.diag_inverse(d_opr, cfg, dinv)
+        net.set_input_scale(dinv)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ds.set_stage(net.stage(), flags=_lib.STAGE_GUESS | _lib.STAGE_PRECOND, precond_iters=1)
+        d_sol.zero_()
+        it, _ = ds.solve(d_rhs, d_sol)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        st = ds.stats()
+        ds.set_stage(None)
+        if rep:  # the first is a warm-up
+            setup_ms.append((t1 - t) * 1e3)
+            solve_ms.append((t2 - t1) * 1e3)
+            its.append(it)
+    # the network's cost per call, and the guess it gives on its own
+    res_buf = torch.zeros_like(dinv)
+    bn = float(torch.linalg.vector_norm(learned.interior_flat(d_rhs, cfg)))
+    e = net.apply(learned.interior_flat(d_rhs, cfg))
+    r_guess = ds.residual(d_rhs, learned.to_ghosted(e, cfg), res_buf) / bn
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(3):
+        net.apply(res_buf)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t) / 3 * 1e3
+    net.close()
+    ds.destroy()
+    res = {"iters": its[0], "solve_ms": round(float(np.median(solve_ms)), 3),
+           "setup_ms": round(float(np.median(setup_ms)), 3), "true_rerr": st["true_rerr"],
+           "stage_calls": st["stage_calls"], "guess_rejected": int(bool(st["fallback"] & 2)),
+           "guess_rel_residual": r_guess, "stage_ms": round(fwd_ms, 3),
+           "training": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in info.items()}}
+    log(f"[bench] two-phase mixed + U-Net: it={res['iters']} solve {res['solve_ms']} ms setup {res['setup_ms']} ms "
+        f"(guess residual {r_guess:.3f}, stage {fwd_ms:.2f} ms)")
+    return res
+
+
+def device_step_leg(args, dev, reps=3):
+    """One pressure-projection step of the two-phase workload (configuration 3's
+    spheres at n^3) with every array in HBM, in the order of
+    examples/falling_drop/src/simulation.f90:298-312 and tpns_class.f90:1565-1716:
+    face densities and the operator (update_laplacian), setup, div of the
+    velocity, rhs = -vol div / dt, solve from zero, shift_p, the pressure
+    gradient, and the velocity correction U -= dt Px / rho_U with P += sol.
+    The velocity is a configuration-1 Gaussian face field. Reports each part
+    and the whole, median of `reps` after a warm-up, and the divergence left
+    after the correction relative to before (the projection's own check)."""
+    import torch
+
+    from nga2_amd import drift, incomp, inputs, learned
+    from nga2_amd._lib import MIXED
+    from nga2_amd.config import Config, uniform
+    from nga2_amd.linsol import DeviceSolve
+    from nga2_amd.proj import DeviceProjection
+
+    n = args.n
+    x = uniform(n)
+    cfg = Config(x, x, x, no=1, xper=True, yper=True, zper=True, name="two_phase_step")
+    met = incomp.Metrics(cfg)
+    proj = DeviceProjection(met)
+    centres, radii = inputs.sphere_set(50, 0.05, 0.15, 3003)
+    vf = drift.sphere_vf_device(n, centres, radii, device=dev)
+    No = cfg.nxo * cfg.nyo * cfg.nzo
+    z = lambda: torch.zeros(No, dtype=torch.float64, device=dev)  # noqa: E731
+    vel = [z() for _ in range(3)]
+    opr = torch.zeros(7 * No, dtype=torch.float64, device=dev)
+    div, rhs, sol, P = z(), z(), z(), z()
+    grads = [z() for _ in range(3)]
+    ds = DeviceSolve(cfg)
+    dt = 1.0
+    parts = {k: [] for k in ("operator", "setup", "rhs", "solve", "shift_pgrad_correct", "step")}
+    its, div_ratio = [], []
+    for rep in range(reps + 1):
+        learned.gaussian_rhs_device(proj, cfg, 8001 + rep, rhs, faces=vel, div=div)  # vel <- Gaussian faces
+        torch.cuda.synchronize()
+        t = [time.perf_counter()]
+        rf = drift.face_densities_device(cfg, vf, 1000.0, 1.0)
+        proj.laplacian(opr, rf)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        ds.setup(opr, maxit=500, rcvg=args.rcvg, precision=MIXED)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        proj.div(*vel, div)
+        proj.rhs(div, 1.0, dt, rhs)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        sol.zero_()
+        it, _ = ds.solve(rhs, sol)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        proj.shift_p(sol)
+        proj.sync(sol)
+        proj.pgrad(sol, *grads)
+        proj.correct(dt, 1.0, grads, vel, rho_faces=rf, P=P, sol=sol)
+        torch.cuda.synchronize()
+        t.append(time.perf_counter())
+        if rep:
+            for k, (a, b) in zip(("operator", "setup", "rhs", "solve", "shift_pgrad_correct"), zip(t, t[1:])):
+                parts[k].append((b - a) * 1e3)
+            parts["step"].append((t[-1] - t[0]) * 1e3)
+            its.append(it)
+            d0 = float(torch.linalg.vector_norm(learned.interior_flat(div, cfg)))
+            for v in vel:
+                proj.sync(v)
+            proj.div(*vel, div)
+            div_ratio.append(float(torch.linalg.vector_norm(learned.interior_flat(div, cfg))) / d0)
+    ds.destroy()
+    proj.destroy()
+    res = {k: round(float(np.median(v)), 3) for k, v in parts.items()}
+    res = {f"{k}_ms": v for k, v in res.items()}
+    res.update({"iters": its[0], "div_after_over_before": max(div_ratio),
+                "workload": f"{n}^3 periodic, 50 spheres r 0.05-0.15 (VF 4^3 sub-cells), density ratio 1000, "
+                            "Gaussian face velocity; mixed precision, rcvg "
+                            f"{args.rcvg:g}; U, V, W, P, opr, rhs, sol resident in HBM throughout"})
+    log(f"[bench] device time step: {res['step_ms']} ms ({res})")
     return res
 
 
@@ -594,6 +753,9 @@ def main():
     two_phase = None
     if not args.no_two_phase:
         two_phase = two_phase_leg(args, dev)
+    step_res = None
+    if not args.no_device_step:
+        step_res = device_step_leg(args, dev)
     sequence = None
     if not args.no_sequence:
         from nga2_amd import learned
@@ -679,6 +841,8 @@ def main():
         out["host_path"] = host_res
     if two_phase:
         out["two_phase_256"] = two_phase
+    if step_res:
+        out["device_time_step"] = step_res
     if sequence:
         out["learned_sequence"] = sequence
     if drift_res:

@@ -153,31 +153,10 @@ class DriftingSpheres:
         return (self.centres + step * self.vel) % 1.0
 
     def rhs_step(self, step):
-        """self.rhs <- the step's right-hand side (ghosted, Fortran order):
-        U, V, W ~ N(0, 1) i.i.d. on the interior faces (torch generator seeded
-        by rhs_seed and the step), per-component mean removed, ghosts synced,
-        div by the device projection, rhs = -vol div / dt (dt = 1), mean
-        removed over the interior. inputs._gaussian_faces is the same recipe
-        drawn with numpy."""
-        import torch
-
-        cfg = self.cfg
-        g = torch.Generator(device=self.dev)
-        g.manual_seed(int(self.rhs_seed) * 1000003 + int(step))
-        no = cfg.no
-        for F in self._faces:
-            v = torch.randn(self.N, generator=g, dtype=torch.float64, device=self.dev)
-            v -= v.mean()
-            F.zero_()
-            F.view(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx] = \
-                v.view(cfg.nz, cfg.ny, cfg.nx)
-            self.proj.sync(F)
-        self.proj.div(*self._faces, self._div)
-        self.proj.rhs(self._div, 1.0, 1.0, self.rhs)
-        torch.cuda.synchronize()
-        ri = self.rhs.view(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx]
-        ri -= ri.mean()
-        return self.rhs
+        """self.rhs <- the step's right-hand side by the configuration-1 recipe
+        (learned.gaussian_rhs_device), seeded by rhs_seed and the step."""
+        return learned.gaussian_rhs_device(self.proj, self.cfg, int(self.rhs_seed) * 1000003 + int(step), self.rhs,
+                                           faces=self._faces, div=self._div)
 
     def operator(self, step):
         """Assemble the step's operator into self.opr (device); returns VF.
@@ -196,107 +175,6 @@ class DriftingSpheres:
         torch.cuda.synchronize()
         self.last_ms = {"vf_ms": (t1 - t0) * 1e3, "laplacian_ms": (time.perf_counter() - t1) * 1e3}
         return vf
-
-    def diag_inverse(self, out):
-        """out (interior, x fastest) <- 1 / opr(1) of the current operator (0 on
-        identity rows)."""
-        import torch
-
-        cfg = self.cfg
-        no = cfg.no
-        d = self.opr.view(cfg.nzo, cfg.nyo, cfg.nxo, 7)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx, 0]
-        d = d.reshape(-1)
-        torch.where(d != 0, 1.0 / torch.where(d != 0, d, torch.ones_like(d)), torch.zeros_like(d), out=out)
-        return out
-
-    def crop(self, a, off, m):
-        """m^3 periodic crop of an interior field (x fastest) at offset (i, j, k);
-        returns a (m, m, m) tensor indexed [k, j, i]."""
-        import torch
-
-        n = self.n
-        ix = [(torch.arange(m, device=self.dev) + int(o)) % n for o in off]
-        a3 = a.view(n, n, n)
-        return a3.index_select(0, ix[2]).index_select(1, ix[1]).index_select(2, ix[0])
-
-    def crop_operator(self, off, m):
-        """The 7 stencil entries of the current operator over an m^3 crop,
-        (7, m, m, m) indexed [s, k, j, i] in the stencil order of stc."""
-        import torch
-
-        cfg = self.cfg
-        no = cfg.no
-        o = self.opr.view(cfg.nzo, cfg.nyo, cfg.nxo, 7)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx, :]
-        n = self.n
-        ix = [(torch.arange(m, device=self.dev) + int(q)) % n for q in off]
-        o = o.index_select(0, ix[2]).index_select(1, ix[1]).index_select(2, ix[0])
-        return o.permute(3, 0, 1, 2).contiguous()
-
-
-def crop_apply(o, y):
-    """A_c y for the operator restricted to a crop (its principal submatrix:
-    couplings to cells outside the crop dropped); o (7, m, m, m) from
-    DriftingSpheres.crop_operator, y (m, m, m) indexed [k, j, i]."""
-    out = o[0] * y
-    out[:, :, :-1] += o[1][:, :, :-1] * y[:, :, 1:]
-    out[:, :, 1:] += o[2][:, :, 1:] * y[:, :, :-1]
-    out[:, :-1, :] += o[3][:, :-1, :] * y[:, 1:, :]
-    out[:, 1:, :] += o[4][:, 1:, :] * y[:, :-1, :]
-    out[:-1] += o[5][:-1] * y[1:]
-    out[1:] += o[6][1:] * y[:-1]
-    return out
-
-
-def crop_loss_grad(u, o, dinv, y, kappa, mask, M=None):
-    """(L, dL/dy) of L = sum(mask (u - kappa D^-1 A_c y)^2) / M, M = sum(mask)
-    by default; dL/dy = -2 kappa / M  A_c D^-1 (mask res), since A_c is
-    symmetric."""
-    if M is None:
-        M = float(mask.sum())
-    res = mask * (u - kappa * dinv * crop_apply(o, y))
-    return float((res * res).sum()) / M, (-2.0 * kappa / M) * crop_apply(o, dinv * res)
-
-
-def train_on_crops(net, samples, kappa, border=16, epochs=6, lr=5e-4, lr_end=5e-5, seed=0, log=None):
-    """Adam over crop samples (u, o, dinv) with u = D^-1 b / s on the crop
-    (s = rms), o the crop's operator and dinv = 1 / o[0]. The loss is the
-    Jacobi-scaled residual the correction leaves, mean over the crop
-    without a border of `border` cells (where the crop boundary and the
-    network's periodic padding differ from the full grid):
-        L = mean_mask (u - kappa D^-1 A_c y)^2,  y = net(u),
-    with gradient dL/dy = -2 kappa / M  A_c D^-1 (mask (u - kappa D^-1 A_c y)),
-    A_c symmetric. Returns (per-epoch mean losses, best epoch); the net is
-    left holding the weights of the epoch with the lowest mean loss."""
-    import torch
-
-    net.set_kappa(kappa)
-    m = samples[0][0].shape[0]
-    mask = torch.zeros((m, m, m), dtype=torch.float64, device=samples[0][0].device)
-    mask[border:m - border, border:m - border, border:m - border] = 1.0
-    M = float(mask.sum())
-    rng = np.random.default_rng(seed)
-    total = epochs * len(samples)
-    k = 0
-    hist = []
-    best = (np.inf, None, -1)
-    for ep in range(epochs):
-        losses = []
-        for i in rng.permutation(len(samples)):
-            u, o, dinv = samples[i]
-            a = lr_end + 0.5 * (lr - lr_end) * (1.0 + np.cos(np.pi * k / max(total - 1, 1)))
-            y = net.forward(u.reshape(-1)).to(torch.float64).view(m, m, m)
-            loss, g = crop_loss_grad(u, o, dinv, y, kappa, mask, M)
-            losses.append(loss)
-            net.backward(g.reshape(-1).to(torch.float32))
-            net.adam_step(float(a))
-            k += 1
-        hist.append(float(np.mean(losses)))
-        if log:
-            log(f"[drift] U-Net epoch {ep}: loss {hist[-1]:.4e}")
-        if hist[-1] < best[0]:
-            best = (hist[-1], net.state(), ep)
-    net.load_state(best[1])
-    return hist, best[2]
 
 
 def run(n=512, steps=100, train_steps=12, nsph=200, drift_cells=0.5, unet=True, crop=128, crops_per_step=6,
@@ -362,21 +240,17 @@ def run(n=512, steps=100, train_steps=12, nsph=200, drift_cells=0.5, unet=True, 
         rec = {"step": step, "fp64": s64, "mixed": smx,
                "diff_mixed": float(torch.linalg.vector_norm(xmxm - x64m)) / nref}
         if unet and step < train_steps:
-            seq.diag_inverse(dinv)
+            learned.diag_inverse(seq.opr, cfg, dinv)
             b = learned.interior_flat(seq.rhs, cfg)
-            for _ in range(crops_per_step):
-                off = gen.integers(0, n, 3)
-                o = seq.crop_operator(off, crop)
-                db = seq.crop(dinv * b, off, crop)
-                s = float(torch.sqrt((db * db).mean()))
-                p = seq.crop(x64m, off, crop)
-                kappas.append(float(torch.sqrt((p * p).mean())) / s)
-                samples.append(((db / s).to(torch.float32).to(torch.float64), o, 1.0 / o[0]))
+            smp, kap = learned.crop_samples(seq.opr, cfg, b, x64m, dinv, crop, crops_per_step, gen)
+            samples += smp
+            kappas += kap
             if step == train_steps - 1:
                 t0 = time.perf_counter()
                 kappa = float(np.median(kappas))
                 net = UNet((crop, crop, crop), seed=0, device=device)
-                hist, best = train_on_crops(net, samples, kappa, epochs=epochs, lr=lr, log=log)
+                hist, best = learned.train_on_crops(net, samples, kappa, epochs=epochs, lr=lr,
+                                                    log=lambda m: log("[drift] " + m))
                 state = net.state()
                 net.close()
                 samples.clear()
@@ -388,7 +262,7 @@ def run(n=512, steps=100, train_steps=12, nsph=200, drift_cells=0.5, unet=True, 
                 log(f"[drift] U-Net trained in {train_info['train_s']:.1f} s, loss {hist[0]:.3e} -> {min(hist):.3e}")
         elif unet and net_full is not None:
             ts = time.perf_counter()
-            seq.diag_inverse(dinv)  # the stage's input scale for this step's operator
+            learned.diag_inverse(seq.opr, cfg, dinv)  # the stage's input scale for this step's operator
             torch.cuda.synchronize()
             dinv_ms = (time.perf_counter() - ts) * 1e3
             # the mixed setup of this step is already in dsmx; the U-Net solve

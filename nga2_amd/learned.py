@@ -308,3 +308,212 @@ def run(n=128, steps=200, train_steps=150, m=6, epochs=6, lr=5e-4, seed=0, preco
     net.close()
     S.close()
     return res
+
+
+# ------------------------------------------------------------------ variable-density operators
+# For a variable-coefficient operator the network is given the Jacobi-scaled
+# residual D^-1 r (D = diag A, the input scale of UNet.set_input_scale). Inside
+# one phase of a two-phase operator D^-1 A is close to (h^2/6) times the
+# constant-coefficient Laplacian whatever the density, so one set of weights
+# serves every density field and the network need not see it separately. The
+# network is trained on crops (a few m^3 windows of full-size solves) with the
+# Jacobi-scaled residual the correction leaves as the loss.
+
+def gaussian_rhs_device(proj, cfg, seed, out, faces=None, div=None):
+    """out (ghosted, Fortran order) <- a right-hand side by the configuration-1
+    recipe: U, V, W ~ N(0, 1) i.i.d. on the interior faces (a torch generator
+    on the device, seeded by `seed`), per-component mean removed, ghosts
+    synced, div by the device projection (ngap_proj_div), rhs = -vol div / dt
+    with dt = 1 (ngap_proj_rhs), mean removed over the interior.
+    inputs._gaussian_faces is the same recipe drawn with numpy."""
+    import torch
+
+    dev = out.device
+    No = cfg.nxo * cfg.nyo * cfg.nzo
+    N = cfg.nx * cfg.ny * cfg.nz
+    faces = faces if faces is not None else [torch.zeros(No, dtype=torch.float64, device=dev) for _ in range(3)]
+    div = div if div is not None else torch.zeros(No, dtype=torch.float64, device=dev)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    no = cfg.no
+    for F in faces:
+        v = torch.randn(N, generator=g, dtype=torch.float64, device=dev)
+        v -= v.mean()
+        F.zero_()
+        F.view(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx] = \
+            v.view(cfg.nz, cfg.ny, cfg.nx)
+        proj.sync(F)
+    proj.div(*faces, div)
+    proj.rhs(div, 1.0, 1.0, out)
+    torch.cuda.synchronize()
+    ri = out.view(cfg.nzo, cfg.nyo, cfg.nxo)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx]
+    ri -= ri.mean()
+    return out
+
+
+def _interior_opr(opr, cfg):
+    no = cfg.no
+    return opr.view(cfg.nzo, cfg.nyo, cfg.nxo, 7)[no:no + cfg.nz, no:no + cfg.ny, no:no + cfg.nx, :]
+
+
+def diag_inverse(opr, cfg, out):
+    """out (interior, x fastest) <- 1 / opr(1) of a ghosted Fortran-ordered
+    operator (0 on identity rows, opr(1) == 0)."""
+    import torch
+
+    d = _interior_opr(opr, cfg)[..., 0].reshape(-1)
+    torch.where(d != 0, 1.0 / torch.where(d != 0, d, torch.ones_like(d)), torch.zeros_like(d), out=out)
+    return out
+
+
+def _crop_index(cfg, off, m, dev):
+    import torch
+
+    return [(torch.arange(m, device=dev) + int(q)) % n for q, n in zip(off, (cfg.nx, cfg.ny, cfg.nz))]
+
+
+def crop_field(a, cfg, off, m):
+    """m^3 periodic window of an interior field (x fastest) at offset (i, j, k),
+    as an (m, m, m) tensor indexed [k, j, i]."""
+    ix = _crop_index(cfg, off, m, a.device)
+    a3 = a.view(cfg.nz, cfg.ny, cfg.nx)
+    return a3.index_select(0, ix[2]).index_select(1, ix[1]).index_select(2, ix[0])
+
+
+def crop_operator(opr, cfg, off, m):
+    """The 7 stencil entries of a ghosted Fortran-ordered operator over an m^3
+    window, (7, m, m, m) indexed [s, k, j, i] in the stencil order
+    (0, +x, -x, +y, -y, +z, -z)."""
+    ix = _crop_index(cfg, off, m, opr.device)
+    o = _interior_opr(opr, cfg)
+    o = o.index_select(0, ix[2]).index_select(1, ix[1]).index_select(2, ix[0])
+    return o.permute(3, 0, 1, 2).contiguous()
+
+
+def crop_apply(o, y):
+    """A_c y for the operator restricted to a window (its principal submatrix:
+    couplings to cells outside the window dropped); o (7, m, m, m) from
+    crop_operator, y (m, m, m) indexed [k, j, i]."""
+    out = o[0] * y
+    out[:, :, :-1] += o[1][:, :, :-1] * y[:, :, 1:]
+    out[:, :, 1:] += o[2][:, :, 1:] * y[:, :, :-1]
+    out[:, :-1, :] += o[3][:, :-1, :] * y[:, 1:, :]
+    out[:, 1:, :] += o[4][:, 1:, :] * y[:, :-1, :]
+    out[:-1] += o[5][:-1] * y[1:]
+    out[1:] += o[6][1:] * y[:-1]
+    return out
+
+
+def crop_loss_grad(u, o, dinv, y, kappa, mask, M=None):
+    """(L, dL/dy) of L = sum(mask (u - kappa D^-1 A_c y)^2) / M, M = sum(mask)
+    by default; dL/dy = -2 kappa / M  A_c D^-1 (mask res), since A_c is
+    symmetric."""
+    if M is None:
+        M = float(mask.sum())
+    res = mask * (u - kappa * dinv * crop_apply(o, y))
+    return float((res * res).sum()) / M, (-2.0 * kappa / M) * crop_apply(o, dinv * res)
+
+
+def crop_samples(opr, cfg, b, x, dinv, m, count, gen):
+    """`count` training windows of one solve: b the right-hand side and x the
+    (mean-free) solution, both interior and x fastest, dinv = 1 / diag A.
+    Returns ([(u, o, dinv_c)], [kappa estimates]) with u = D^-1 b / s on the
+    window (s = its rms, rounded through FP32 as the network sees it), o the
+    window's operator and the ratio rms(x) / s per window."""
+    import torch
+
+    samples, kappas = [], []
+    for _ in range(count):
+        off = gen.integers(0, (cfg.nx, cfg.ny, cfg.nz), 3)
+        o = crop_operator(opr, cfg, off, m)
+        db = crop_field(dinv * b, cfg, off, m)
+        s = float(torch.sqrt((db * db).mean()))
+        p = crop_field(x, cfg, off, m)
+        kappas.append(float(torch.sqrt((p * p).mean())) / s)
+        samples.append(((db / s).to(torch.float32).to(torch.float64), o, 1.0 / o[0]))
+    return samples, kappas
+
+
+def train_on_crops(net, samples, kappa, border=16, epochs=6, lr=5e-4, lr_end=5e-5, seed=0, log=None):
+    """Adam over window samples (u, o, dinv) of crop_samples. The loss is the
+    Jacobi-scaled residual the correction leaves, mean over the window
+    without a border of `border` cells (where the window's edge and the
+    network's periodic padding differ from the full grid):
+        L = mean_mask (u - kappa D^-1 A_c y)^2,  y = net(u),
+    with gradient dL/dy = -2 kappa / M  A_c D^-1 (mask (u - kappa D^-1 A_c y)),
+    A_c symmetric. Returns (per-epoch mean losses, best epoch); the net is
+    left holding the weights of the epoch with the lowest mean loss."""
+    import torch
+
+    net.set_kappa(kappa)
+    m = samples[0][0].shape[0]
+    mask = torch.zeros((m, m, m), dtype=torch.float64, device=samples[0][0].device)
+    mask[border:m - border, border:m - border, border:m - border] = 1.0
+    M = float(mask.sum())
+    rng = np.random.default_rng(seed)
+    total = epochs * len(samples)
+    k = 0
+    hist = []
+    best = (np.inf, None, -1)
+    for ep in range(epochs):
+        losses = []
+        for i in rng.permutation(len(samples)):
+            u, o, dinv = samples[i]
+            a = lr_end + 0.5 * (lr - lr_end) * (1.0 + np.cos(np.pi * k / max(total - 1, 1)))
+            y = net.forward(u.reshape(-1)).to(torch.float64).view(m, m, m)
+            loss, g = crop_loss_grad(u, o, dinv, y, kappa, mask, M)
+            losses.append(loss)
+            net.backward(g.reshape(-1).to(torch.float32))
+            net.adam_step(float(a))
+            k += 1
+        hist.append(float(np.mean(losses)))
+        if log:
+            log(f"U-Net epoch {ep}: loss {hist[-1]:.4e}")
+        if hist[-1] < best[0]:
+            best = (hist[-1], net.state(), ep)
+    net.load_state(best[1])
+    return hist, best[2]
+
+
+def train_for_operator(ds, proj, opr, cfg, seeds, crop=128, crops_per_solve=8, epochs=6, lr=5e-4, log=None,
+                       device="cuda:0"):
+    """Train the network for one operator from its own solves: for every seed
+    a configuration-1 right-hand side (gaussian_rhs_device) is solved by ds
+    (set up on opr) from a zero guess, and crops of (rhs, solution) become
+    training windows (crop_samples, train_on_crops). Returns
+    (state, info): the best epoch's weights with kappa, and the training
+    record. The net is trained on crop^3 windows; load the state into a
+    full-size UNet to apply it."""
+    import torch
+
+    from .unet import UNet
+
+    No = cfg.nxo * cfg.nyo * cfg.nzo
+    N = cfg.nx * cfg.ny * cfg.nz
+    dev = opr.device
+    rhs = torch.zeros(No, dtype=torch.float64, device=dev)
+    x = torch.zeros(No, dtype=torch.float64, device=dev)
+    dinv = diag_inverse(opr, cfg, torch.zeros(N, dtype=torch.float64, device=dev))
+    gen = np.random.default_rng(91)
+    samples, kappas, its = [], [], []
+    t0 = time.perf_counter()
+    for sd in seeds:
+        gaussian_rhs_device(proj, cfg, sd, rhs)
+        x.zero_()
+        it, _ = ds.solve(rhs, x)
+        its.append(it)
+        smp, kap = crop_samples(opr, cfg, interior_flat(rhs, cfg), _mean_free(interior_flat(x, cfg)), dinv, crop,
+                                crops_per_solve, gen)
+        samples += smp
+        kappas += kap
+    t1 = time.perf_counter()
+    kappa = float(np.median(kappas))
+    net = UNet((crop, crop, crop), seed=0, device=device)
+    try:
+        hist, best = train_on_crops(net, samples, kappa, epochs=epochs, lr=lr, log=log)
+        state = net.state()
+    finally:
+        net.close()
+    info = {"solves": len(seeds), "solve_iters": its, "windows": len(samples), "crop": crop, "kappa": kappa,
+            "loss_history": hist, "best_epoch": best, "collect_s": t1 - t0, "train_s": time.perf_counter() - t1}
+    return state, info

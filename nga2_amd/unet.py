@@ -316,8 +316,7 @@ class UNet:
         """Write the weights and kappa to an .npz file together with the
         SHA-256 of their bytes (see state_digest); load() refuses a file whose
         digest does not match."""
-        st = self.state()
-        np.savez(path, sha256=np.frombuffer(state_digest(st).encode(), dtype=np.uint8), **st)
+        write_state(path, self.state())
 
     def load(self, path):
         self.load_state(read_state(path))
@@ -336,6 +335,12 @@ def state_digest(st):
         h.update(a.tobytes())
     h.update(np.asarray(st["kappa"], dtype=np.float64).reshape(-1).tobytes())
     return h.hexdigest()
+
+
+def write_state(path, st):
+    """Write a state dict (layer name -> float32 weights, "kappa") to an .npz
+    file with its digest."""
+    np.savez(path, sha256=np.frombuffer(state_digest(st).encode(), dtype=np.uint8), **st)
 
 
 def read_state(path):

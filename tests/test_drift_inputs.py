@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 import torch
 
-from nga2_amd import drift, inputs
+from nga2_amd import drift, inputs, learned
 from nga2_amd.config import Config, uniform
 
 
@@ -111,12 +111,12 @@ def test_crop_apply_is_the_principal_submatrix():
     crop = lambda a: a[np.ix_(ix[2], ix[1], ix[0])]  # noqa: E731
     oc = torch.from_numpy(np.stack([crop(o[s]) for s in range(7)]))
     yc = torch.from_numpy(crop(x))
-    got = drift.crop_apply(oc, yc).numpy()
+    got = learned.crop_apply(oc, yc).numpy()
     assert np.allclose(got[1:-1, 1:-1, 1:-1], crop(full)[1:-1, 1:-1, 1:-1], rtol=1e-13, atol=1e-13)
     # a field that vanishes inside the crop sees no coupling from outside
     only_out = x.copy()
     only_out[np.ix_(ix[2], ix[1], ix[0])] = 0.0
-    assert np.abs(drift.crop_apply(oc, torch.from_numpy(crop(only_out))).numpy()).max() == 0.0
+    assert np.abs(learned.crop_apply(oc, torch.from_numpy(crop(only_out))).numpy()).max() == 0.0
 
 
 def test_crop_loss_gradient_matches_autograd():
@@ -135,8 +135,8 @@ def test_crop_loss_gradient_matches_autograd():
     mask = torch.zeros((m, m, m), dtype=torch.float64)
     mask[2:-2, 2:-2, 2:-2] = 1.0
     kappa = 0.7
-    loss, grad = drift.crop_loss_grad(u, o, dinv, y.detach(), kappa, mask)
-    res = mask * (u - kappa * dinv * drift.crop_apply(o, y))
+    loss, grad = learned.crop_loss_grad(u, o, dinv, y.detach(), kappa, mask)
+    res = mask * (u - kappa * dinv * learned.crop_apply(o, y))
     L = (res * res).sum() / mask.sum()
     L.backward()
     Lv = float(L.detach())

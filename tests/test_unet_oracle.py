@@ -140,3 +140,35 @@ def test_gradients_match_finite_differences():
         fm = float(U.forward(Wm, r, shape, quantize=False).numpy() @ dy)
         fd = (fp - fm) / (2 * h)
         assert abs(G[name][idx] - fd) <= 1e-6 * max(1.0, abs(fd)), (name, G[name][idx], fd)
+
+
+def test_weight_file_checksum(tmp_path):
+    """U-Net weight files carry the SHA-256 of their contents: a file read back
+    unchanged loads; one changed weight, a missing digest or a missing layer
+    is refused."""
+    from nga2_amd import unet
+
+    st = unet.init_weights(seed=4)
+    st["kappa"] = np.array([0.25])
+    p = tmp_path / "w.npz"
+    unet.write_state(p, st)
+    back = unet.read_state(p)
+    assert all(np.array_equal(back[k], np.asarray(st[k])) for k in st)
+    bad = dict(st)
+    bad["d1"] = np.asarray(st["d1"]).copy()
+    bad["d1"][17] = np.nextafter(bad["d1"][17], np.float32(1))  # one ulp
+    with np.load(p) as z:
+        files = {k: z[k] for k in z.files}
+    files["d1"] = bad["d1"]
+    q = tmp_path / "tampered.npz"
+    np.savez(q, **files)
+    with pytest.raises(ValueError, match="checksum"):
+        unet.read_state(q)
+    del files["sha256"]
+    np.savez(q, **files)
+    with pytest.raises(ValueError, match="sha256"):
+        unet.read_state(q)
+    files = {k: v for k, v in st.items() if k != "b1"}
+    np.savez(q, sha256=np.frombuffer(b"0" * 64, dtype=np.uint8), **files)
+    with pytest.raises(ValueError, match="missing"):
+        unet.read_state(q)
