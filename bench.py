@@ -81,7 +81,7 @@ def parse():
     ap.add_argument("--seq-n", type=int, default=128)
     ap.add_argument("--seq-m", type=int, default=8)
     ap.add_argument("--seq-epochs", type=int, default=6)
-    ap.add_argument("--kernel-only", choices=["rbgs", "spmv", "restrict", "prolong"], default=None,
+    ap.add_argument("--kernel-only", choices=sorted(KERNEL_IDS), default=None,
                     help="time one fine-level kernel (for rocprofv3 summaries and counters) and exit")
     return ap.parse_args()
 
@@ -125,7 +125,8 @@ def profiled_traffic(name):
     return None
 
 
-KERNEL_IDS = {"rbgs": (1, 1), "spmv": (0, 0), "restrict": (2, 1), "prolong": (3, 1)}  # (id, precision)
+KERNEL_IDS = {"rbgs": (1, 1), "spmv": (0, 0), "restrict": (2, 1), "prolong": (3, 1),  # (id, precision)
+              "sweep2": (6, 1), "sweep2_zero": (7, 1), "rbgs64": (1, 0)}
 
 
 def kernel_only(args):
@@ -674,33 +675,45 @@ def main():
     if not args.no_host_path:
         host_res = host_path_leg(args, P, 1e3 * mixed["elapsed"] / args.steps)
 
-    # roofline of the dominant kernel (fine-level FP32 red/black GS colour pass)
+    # Roofline of the kernel that runs on the fine level of the headline (mixed) cycle: its
+    # post-smoothing pair k_sweep2<float,16,false>, the largest fine-level launch. "achieved"
+    # counts the algorithmic bytes of its one pass over the arrays (7 x 4 B per cell);
+    # "sweep_equiv_gbs" the bytes of the two single-sweep launches it replaces. The other
+    # fine-level kernels, the single sweep k_sweep<float> (levels 1-17 of the cycle) and the
+    # FP64 sweep and SpMV of the plain method are timed beside it.
     hk = ds_mixed.hooks()
     rb_ms, rb_bytes = hk.time_kernel(_lib.K_RBGS, prec=1, reps=args.roofline_reps)
+    r64_ms, r64_bytes = hk.time_kernel(_lib.K_RBGS, prec=0, reps=args.roofline_reps)
     sp_ms, sp_bytes = hk.time_kernel(_lib.K_SPMV, prec=0, reps=args.roofline_reps)
     p2_ms, p2_bytes = hk.time_kernel(_lib.K_SWEEP2, prec=1, reps=args.roofline_reps)
     p0_ms, p0_bytes = hk.time_kernel(_lib.K_SWEEP2_ZERO, prec=1, reps=args.roofline_reps)
-    rb_gbs = rb_bytes / (rb_ms * 1e-3) / 1e9
-    sp_gbs = sp_bytes / (sp_ms * 1e-3) / 1e9
     profiled = args.n == 256 and args.workload == "channel_spectral"
-    rb_traffic = profiled_traffic("sweep_fp32_channel256.txt") if profiled else None
-    sp_traffic = profiled_traffic("spmv_fp64_channel256.txt") if profiled else None
-    roofline = {"bound": "hbm", "achieved": round(rb_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(rb_gbs / HBM_PEAK_GBS, 4), "traffic": rb_traffic,
-                "traffic_source": "profiles/sweep_fp32_channel256.txt (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE; "
-                                  "copy-calibrated)" if rb_traffic else None,
-                "kernel": "k_sweep<float> fused red/black Gauss-Seidel sweep, fine level (7 arrays x 4 B per cell)",
-                "avg_ms": rb_ms,
-                "algorithmic_bytes": rb_bytes,
-                "spmv_fp64": {"achieved": round(sp_gbs, 1), "frac": round(sp_gbs / HBM_PEAK_GBS, 4), "avg_ms": sp_ms,
-                              "algorithmic_bytes": sp_bytes, "traffic": sp_traffic}}
-    # The mixed cycle's fine level runs its sweeps as fused pairs: two sweeps per pass over the arrays.
-    # "achieved" counts the bytes of one pass; "sweep_equiv" the bytes two single sweeps would move.
-    for key, ms, nb, sweep_b in (("sweep2_fp32", p2_ms, p2_bytes, 2 * rb_bytes),
-                                 ("sweep2_zero_fp32", p0_ms, p0_bytes, 6.0 / 7.0 * rb_bytes + rb_bytes)):
+
+    def entry(ms, nb, prof, kernel, sweep_b=None):
         gbs = nb / (ms * 1e-3) / 1e9
-        roofline[key] = {"achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_ms": ms,
-                         "algorithmic_bytes": nb, "sweep_equiv_gbs": round(sweep_b / (ms * 1e-3) / 1e9, 1)}
+        e = {"kernel": kernel, "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "avg_ms": ms,
+             "algorithmic_bytes": nb, "traffic": profiled_traffic(prof) if profiled else None,
+             "traffic_source": f"profiles/{prof}" if profiled else None}
+        if sweep_b is not None:
+            e["sweep_equiv_gbs"] = round(sweep_b / (ms * 1e-3) / 1e9, 1)
+            e["sweep_equiv_frac"] = round(sweep_b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+        return e
+
+    post = entry(p2_ms, p2_bytes, "sweep2_post_fp32_channel256.txt",
+                 "k_sweep2<float,16,false>: the fine level's two post-smoothing red/black GS sweeps in one pass",
+                 2 * rb_bytes)
+    roofline = {"bound": "hbm", "achieved": post["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": post["frac"], "traffic": post["traffic"]}
+    roofline.update({k: v for k, v in post.items() if k not in roofline})
+    roofline["sweep2_zero_fp32"] = entry(p0_ms, p0_bytes, "sweep2_pre_fp32_channel256.txt",
+                                         "k_sweep2<float,16,true>: the fine level's pre-smoothing pair from a zero iterate",
+                                         6.0 / 7.0 * rb_bytes + rb_bytes)
+    roofline["sweep_fp32"] = entry(rb_ms, rb_bytes, "sweep_fp32_channel256.txt",
+                                   "k_sweep<float>: one fused red/black GS sweep (runs on levels 1-17 of the cycle)")
+    roofline["sweep_fp64"] = entry(r64_ms, r64_bytes, "sweep_fp64_channel256.txt",
+                                   "k_sweep<double>: the plain method's sweep (RelaxType 2), fine level")
+    roofline["spmv_fp64"] = entry(sp_ms, sp_bytes, "spmv_fp64_channel256.txt",
+                                  "k_stencil_rw<double,2,APPLY>: the FP64 SpMV of the outer PCG")
 
     # CPU baseline + FP64 oracle error
     cpu = None

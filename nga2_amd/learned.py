@@ -404,22 +404,27 @@ def crop_apply(o, y):
     return out
 
 
-def crop_loss_grad(u, o, dinv, y, kappa, mask, M=None):
-    """(L, dL/dy) of L = sum(mask (u - kappa D^-1 A_c y)^2) / M, M = sum(mask)
-    by default; dL/dy = -2 kappa / M  A_c D^-1 (mask res), since A_c is
-    symmetric."""
+def crop_loss_grad(w, o, c, y, kappa, mask, M=None):
+    """(L, dL/dy) of the residual a correction leaves on a window, in units of
+    the window's rms residual t: with the network input u = D^-1 r / s and
+    output y, the correction is e = s kappa y and
+        (r - A e) / t = w - c kappa A_c y,   w = r / t,  c = s / t,
+    L = sum(mask (w - c kappa A_c y)^2) / M (M = sum(mask) by default) and
+    dL/dy = -2 c kappa / M  A_c (mask res), since A_c is symmetric. This is
+    the 2-norm residual by which the solver accepts a guess."""
     if M is None:
         M = float(mask.sum())
-    res = mask * (u - kappa * dinv * crop_apply(o, y))
-    return float((res * res).sum()) / M, (-2.0 * kappa / M) * crop_apply(o, dinv * res)
+    res = mask * (w - (c * kappa) * crop_apply(o, y))
+    return float((res * res).sum()) / M, (-2.0 * c * kappa / M) * crop_apply(o, res)
 
 
 def crop_samples(opr, cfg, b, x, dinv, m, count, gen):
     """`count` training windows of one solve: b the right-hand side and x the
     (mean-free) solution, both interior and x fastest, dinv = 1 / diag A.
-    Returns ([(u, o, dinv_c)], [kappa estimates]) with u = D^-1 b / s on the
-    window (s = its rms, rounded through FP32 as the network sees it), o the
-    window's operator and the ratio rms(x) / s per window."""
+    Returns ([(u, o, w, c)], [kappa estimates]): on each window u = D^-1 b / s
+    (s its rms, rounded through FP32 as the network sees it), o the window's
+    operator, w = b / t (t = rms of b on the window), c = s / t, and the
+    ratio rms(x) / s."""
     import torch
 
     samples, kappas = [], []
@@ -427,22 +432,22 @@ def crop_samples(opr, cfg, b, x, dinv, m, count, gen):
         off = gen.integers(0, (cfg.nx, cfg.ny, cfg.nz), 3)
         o = crop_operator(opr, cfg, off, m)
         db = crop_field(dinv * b, cfg, off, m)
+        bw = crop_field(b, cfg, off, m)
         s = float(torch.sqrt((db * db).mean()))
+        t = float(torch.sqrt((bw * bw).mean()))
         p = crop_field(x, cfg, off, m)
         kappas.append(float(torch.sqrt((p * p).mean())) / s)
-        samples.append(((db / s).to(torch.float32).to(torch.float64), o, 1.0 / o[0]))
+        samples.append(((db / s).to(torch.float32).to(torch.float64), o, bw / t, s / t))
     return samples, kappas
 
 
 def train_on_crops(net, samples, kappa, border=16, epochs=6, lr=5e-4, lr_end=5e-5, seed=0, log=None):
-    """Adam over window samples (u, o, dinv) of crop_samples. The loss is the
-    Jacobi-scaled residual the correction leaves, mean over the window
-    without a border of `border` cells (where the window's edge and the
-    network's periodic padding differ from the full grid):
-        L = mean_mask (u - kappa D^-1 A_c y)^2,  y = net(u),
-    with gradient dL/dy = -2 kappa / M  A_c D^-1 (mask (u - kappa D^-1 A_c y)),
-    A_c symmetric. Returns (per-epoch mean losses, best epoch); the net is
-    left holding the weights of the epoch with the lowest mean loss."""
+    """Adam over window samples (u, o, w, c) of crop_samples. The loss is the
+    residual the correction leaves (crop_loss_grad), mean over the window
+    without a border of `border` cells, where the window's edge and the
+    network's periodic padding differ from the full grid. Returns (per-epoch
+    mean losses, best epoch); the net is left holding the weights of the
+    epoch with the lowest mean loss."""
     import torch
 
     net.set_kappa(kappa)
@@ -458,10 +463,10 @@ def train_on_crops(net, samples, kappa, border=16, epochs=6, lr=5e-4, lr_end=5e-
     for ep in range(epochs):
         losses = []
         for i in rng.permutation(len(samples)):
-            u, o, dinv = samples[i]
+            u, o, w, c = samples[i]
             a = lr_end + 0.5 * (lr - lr_end) * (1.0 + np.cos(np.pi * k / max(total - 1, 1)))
             y = net.forward(u.reshape(-1)).to(torch.float64).view(m, m, m)
-            loss, g = crop_loss_grad(u, o, dinv, y, kappa, mask, M)
+            loss, g = crop_loss_grad(w, o, c, y, kappa, mask, M)
             losses.append(loss)
             net.backward(g.reshape(-1).to(torch.float32))
             net.adam_step(float(a))

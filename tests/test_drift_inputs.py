@@ -120,25 +120,44 @@ def test_crop_apply_is_the_principal_submatrix():
 
 
 def test_crop_loss_gradient_matches_autograd():
-    # the training gradient of the Jacobi-scaled residual loss against torch
-    # autograd (a sign or transpose slip fails it)
+    # the training gradient of the window residual loss against torch
+    # autograd (a sign, scale or transpose slip fails it)
     m = 10
     o = torch.from_numpy(_periodic_operator(m, 8)).clone()
     for s, (ax, sl) in enumerate(((2, -1), (2, 0), (1, -1), (1, 0), (0, -1), (0, 0))):
         idx = [slice(None)] * 3
         idx[ax] = sl
-        o[1 + s][tuple(idx)] = 0.0  # the crop's outside couplings are absent
+        o[1 + s][tuple(idx)] = 0.0  # the window's outside couplings are absent
     g = torch.Generator().manual_seed(1)
-    u = torch.randn((m, m, m), generator=g, dtype=torch.float64)
+    w = torch.randn((m, m, m), generator=g, dtype=torch.float64)
     y = torch.randn((m, m, m), generator=g, dtype=torch.float64, requires_grad=True)
-    dinv = 1.0 / o[0]
     mask = torch.zeros((m, m, m), dtype=torch.float64)
     mask[2:-2, 2:-2, 2:-2] = 1.0
-    kappa = 0.7
-    loss, grad = learned.crop_loss_grad(u, o, dinv, y.detach(), kappa, mask)
-    res = mask * (u - kappa * dinv * learned.crop_apply(o, y))
+    kappa, c = 0.7, 3.1
+    loss, grad = learned.crop_loss_grad(w, o, c, y.detach(), kappa, mask)
+    res = mask * (w - c * kappa * learned.crop_apply(o, y))
     L = (res * res).sum() / mask.sum()
     L.backward()
     Lv = float(L.detach())
     assert abs(loss - Lv) <= 1e-12 * abs(Lv)
     assert torch.allclose(grad, y.grad, rtol=1e-10, atol=1e-14)
+
+
+def test_crop_loss_is_the_guess_residual():
+    # with e = s kappa y on the full periodic grid (a window that is the whole
+    # grid, no border, wrap couplings included through a periodic operator
+    # built as the window's own), the loss is ||b - A e||^2 / ||b||^2 * N
+    n = 8
+    o = torch.from_numpy(_periodic_operator(n, 5))
+    g = torch.Generator().manual_seed(2)
+    b = torch.randn((n, n, n), generator=g, dtype=torch.float64)
+    y = torch.randn((n, n, n), generator=g, dtype=torch.float64)
+    dinv = 1.0 / o[0]
+    s = float(torch.sqrt(((dinv * b) ** 2).mean()))
+    t = float(torch.sqrt((b * b).mean()))
+    kappa = 0.3
+    e = s * kappa * y
+    mask = torch.ones((n, n, n), dtype=torch.float64)
+    loss, _ = learned.crop_loss_grad(b / t, o, s / t, y, kappa, mask)
+    r = b - learned.crop_apply(o, e)
+    assert abs(loss - float((r * r).sum() / (b * b).sum())) <= 1e-12
