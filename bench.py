@@ -68,6 +68,7 @@ def parse():
     ap.add_argument("--no-two-phase", action="store_true", help="skip the 256^3 two-phase spheres leg")
     ap.add_argument("--no-two-phase-unet", action="store_true", help="two-phase leg without the U-Net stage")
     ap.add_argument("--no-device-step", action="store_true", help="skip the device-resident projection step leg")
+    ap.add_argument("--no-config1", action="store_true", help="skip the 64^3 configuration-1 leg")
     ap.add_argument("--no-sequence", action="store_true", help="skip the learned-guess sequence leg")
     ap.add_argument("--no-drift", action="store_true", help="skip the drifting-spheres two-phase sequence leg")
     ap.add_argument("--no-fourier", action="store_true", help="skip the fourier3d (FFT direct solve) leg")
@@ -405,6 +406,74 @@ def device_step_leg(args, dev, reps=3):
                             "Gaussian face velocity; mixed precision, rcvg "
                             f"{args.rcvg:g}; U, V, W, P, opr, rhs, sol resident in HBM throughout"})
     log(f"[bench] device time step: {res['step_ms']} ms ({res})")
+    return res
+
+
+def config1_leg(args, dev, solves=10):
+    """Configuration 1 at its size: the 64^3 triply periodic Gaussian case
+    (examples/hit/src/simulation.f90:108-116), FP64 and mixed on the GPU (device
+    entry points, median of `solves`) against the FP64 CPU oracle (median of 5
+    after a warm-up, the box's threads) and the FFT closed form of the uniform
+    7-point operator (src/solver/fourier3d_class.f90:1103-1196)."""
+    import torch
+
+    from nga2_amd import inputs
+    from nga2_amd._lib import FP64, MIXED
+    from nga2_amd.linsol import DeviceSolve
+    from oracle.orc import Oracle, interior, num_threads
+
+    P = inputs.periodic_gaussian(n=64)
+    cfg = P.cfg
+    n = cfg.nx
+    b = interior(P.rhs, cfg)
+    # the closed form: -vol lap x = b with eigenvalues vol * sum_d (2 - 2 cos(2 pi k_d / n)) / h^2
+    k = np.arange(n)
+    lam1 = (1.0 / n) * (2.0 - 2.0 * np.cos(2.0 * np.pi * k / n))
+    lam = lam1[:, None, None] + lam1[None, :, None] + lam1[None, None, :]
+    lam[0, 0, 0] = 1.0
+    X = np.fft.fftn(b.reshape(n, n, n)) / lam
+    X[0, 0, 0] = 0.0
+    xc = np.fft.ifftn(X).real.ravel()
+
+    def err(x, ref):
+        x = x - x.mean()
+        r = ref - ref.mean()
+        return float(np.linalg.norm(x - r) / np.linalg.norm(r))
+
+    res = {"workload": "periodic_gaussian_64: 64^3 triply periodic, uniform, Gaussian faces seed 1001, "
+                       f"rcvg {args.rcvg:g}, zero guess"}
+    o = Oracle.from_cfg(cfg, P.opr)
+    times = []
+    for rep in range(6):
+        t = time.perf_counter()
+        xr, st_r = o.solve(b, rcvg=args.rcvg, maxit=500)
+        if rep:
+            times.append(time.perf_counter() - t)
+    o.close()
+    res["cpu_oracle"] = {"iters": st_r["it"], "solve_ms": round(1e3 * float(np.median(times)), 3),
+                         "threads": num_threads(), "err_vs_closed_form": err(xr, xc)}
+    d_opr = torch.from_numpy(np.ascontiguousarray(P.opr.ravel(order="F"))).to(dev)
+    d_rhs = torch.from_numpy(np.ascontiguousarray(P.rhs.ravel(order="F"))).to(dev)
+    d_sol = torch.zeros_like(d_rhs)
+    for name, prec in (("fp64", FP64), ("mixed", MIXED)):
+        ds = DeviceSolve(cfg)
+        ds.setup(d_opr, maxit=500, rcvg=args.rcvg, precision=prec)
+        ms = []
+        for rep in range(solves + 1):
+            d_sol.zero_()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            it, _ = ds.solve(d_rhs, d_sol)
+            torch.cuda.synchronize()
+            if rep:
+                ms.append((time.perf_counter() - t) * 1e3)
+        st = ds.stats()
+        ds.destroy()
+        x = interior(d_sol.cpu().numpy().reshape(cfg.shape, order="F"), cfg)
+        res[name] = {"iters": it, "solve_ms": round(float(np.median(ms)), 3), "true_rerr": st["true_rerr"],
+                     "err_vs_fp64_oracle": err(x, xr), "err_vs_closed_form": err(x, xc)}
+    res["speedup_fp64_gpu_vs_cpu"] = round(res["cpu_oracle"]["solve_ms"] / res["fp64"]["solve_ms"], 2)
+    log(f"[bench] configuration 1 (64^3): {res}")
     return res
 
 
@@ -769,6 +838,9 @@ def main():
     step_res = None
     if not args.no_device_step:
         step_res = device_step_leg(args, dev)
+    c1_res = None
+    if not args.no_cpu and not args.no_config1:
+        c1_res = config1_leg(args, dev)
     sequence = None
     if not args.no_sequence:
         from nga2_amd import learned
@@ -856,6 +928,8 @@ def main():
         out["two_phase_256"] = two_phase
     if step_res:
         out["device_time_step"] = step_res
+    if c1_res:
+        out["config1_64"] = c1_res
     if sequence:
         out["learned_sequence"] = sequence
     if drift_res:
