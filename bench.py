@@ -196,7 +196,7 @@ def host_path_leg(args, P, device_ms):
            "ratio_to_device_path": round(wall / device_ms, 3),
            "host_bytes_gb": round(gb, 3),
            "note": "ngap_solve with pageable host arrays (rhs, guess in; sol out, ghosted), median of 5; "
-                   "copies staged through pinned chunks on 8 host threads, an all-zero guess chunk cleared "
+                   "copies staged through pinned chunks on 4 host threads, an all-zero guess chunk cleared "
                    "on the device instead of copied"}
     log(f"[bench] host path: {res['value']:.2f} solves/s, copy {res['copy_ms']:.1f} ms, "
         f"{res['ratio_to_device_path']:.2f}x the device path")

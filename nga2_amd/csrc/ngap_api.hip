@@ -563,7 +563,7 @@ double now_ms() {
 // a contiguous part of the array, two pinned chunks and a HIP stream, and copy
 // chunk by chunk between pageable and pinned memory while the DMA engine moves
 // the other chunk of the pair.
-constexpr int STG_THREADS = 8;
+constexpr int STG_THREADS = 4;
 constexpr size_t STG_CHUNK = (size_t)4 << 20;  // bytes per pinned chunk
 struct Staging {
     char *pin = nullptr;  // 2 * STG_THREADS chunks
