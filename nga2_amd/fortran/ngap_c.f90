@@ -24,6 +24,7 @@ module ngap_c
    ! status of the last solve
    integer, parameter, public :: ngap_status_converged=0
    integer, parameter, public :: ngap_status_maxit=1
+   integer, parameter, public :: ngap_status_stagnated=2
 
    type, bind(C), public :: ngap_grid_t
       integer(c_int) :: nx,ny,nz

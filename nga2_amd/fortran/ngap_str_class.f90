@@ -178,7 +178,8 @@ contains
       prec='fp64'
       if (this%stats%precision.eq.ngap_mixed) prec='mixed'
       state='converged'
-      if (this%stats%status.ne.0) state='at maxit'
+      if (this%stats%status.eq.ngap_status_maxit) state='at maxit'
+      if (this%stats%status.eq.ngap_status_stagnated) state='stagnated'
       line=''
       write(line(1),'("ngap_str ",a," on ",a,": ",a," (",a,"), ",i0," multigrid levels")') &
       &     trim(this%name),trim(this%cfg%name),trim(meth),trim(prec),this%stats%nlevels
