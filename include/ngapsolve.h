@@ -106,6 +106,11 @@ typedef struct ngap_opts {
     int verbose;      /* 0 quiet, 1 one line per solve on stderr */
     int graphs;       /* replay the V-cycle as a HIP graph, default 1 */
     int kdim;         /* GMRES restart length (HYPRE_StructGMRESSetKDim; HYPRE's default 5) */
+    int skip_relax;   /* 1 (default): on an isotropic fine operator (at most 10% of its cells with
+                         one direction's couplings more than twice another's), smooth only the coarse
+                         levels whose couplings are isotropic (max over the directions of sum |a_d|
+                         <= 2 x min); the others restrict and interpolate without smoothing, after
+                         HYPRE PFMG's SkipRelax (on by default there). 0: smooth every level */
 } ngap_opts;
 
 typedef struct ngap_stats {
@@ -118,6 +123,8 @@ typedef struct ngap_stats {
                             bit 1: the correction stage was dropped (guess rejected or breakdown) */
     int restarts;        /* residual replacements */
     int stage_calls;     /* calls of the correction stage (ngap_set_stage) in the last solve */
+    int skip_relax;      /* 1: the last setup skipped relaxation on some levels (ngap_opts.skip_relax) */
+    int relaxed_levels;  /* levels the V-cycle smooths (the coarsest, solved, included) */
     double rerr;         /* relative 2-norm of the recursive residual at exit */
     double true_rerr;    /* ||b - A x||_2/||b||_2 with the exact 7-array opr */
     double aerr;         /* ||b - A x||_2 */

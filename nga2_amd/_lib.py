@@ -42,12 +42,14 @@ class Grid(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("precision", C.c_int), ("nu_pre", C.c_int), ("nu_post", C.c_int), ("coarsest_max", C.c_int),
-                ("check_finite", C.c_int), ("verbose", C.c_int), ("graphs", C.c_int), ("kdim", C.c_int)]
+                ("check_finite", C.c_int), ("verbose", C.c_int), ("graphs", C.c_int), ("kdim", C.c_int),
+                ("skip_relax", C.c_int)]
 
 
 class Stats(C.Structure):
     _fields_ = [("status", C.c_int), ("it", C.c_int), ("nlevels", C.c_int), ("coarse_singular", C.c_int),
                 ("precision", C.c_int), ("fallback", C.c_int), ("restarts", C.c_int), ("stage_calls", C.c_int),
+                ("skip_relax", C.c_int), ("relaxed_levels", C.c_int),
                 ("rerr", C.c_double), ("true_rerr", C.c_double), ("aerr", C.c_double),
                 ("cnorm_ratio", C.c_double), ("asym", C.c_double), ("setup_ms", C.c_double),
                 ("solve_ms", C.c_double), ("copy_ms", C.c_double), ("cells", C.c_longlong),

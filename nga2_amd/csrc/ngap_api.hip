@@ -767,6 +767,43 @@ int read_scal(ngap_t *h) {
     return 0;
 }
 
+// Which levels the V-cycle smooths (skipped relaxation, after PFMG's SkipRelax,
+// with this project's rule): with skip set, a level l with 0 < l < nlev-1 is
+// smoothed only when its couplings are isotropic: over the directions with
+// n_d > 1 and s_d = sum |a_d| > 0 (the strengths of the coarsening choice),
+// max s_d <= 2 min s_d (with a 1e-6 relative margin, so a ratio of 2 up to the
+// rounding of the sums counts as isotropic in any summation order), or fewer
+// than two such directions. Semicoarsening an
+// isotropic grid in x, y, z takes the strengths (1,1,1) -> (1/2,2,2) ->
+// (1,1,4) -> (2,2,2), so every third level is smoothed, as in full
+// coarsening. A skipped level restricts its right-hand side unchanged on the
+// way down (v = 0) and takes v = P v_coarse on the way up. The finest level is
+// always smoothed and the coarsest always solved.
+// Skipped relaxation is for isotropic operators: PFMG's semicoarsening
+// handles anisotropy by coarsening the strong direction, and smoothing only
+// every few levels loses that. The gate opens when at most 10% of the fine
+// level's cells are anisotropic by k_aniso_count's measure (a stretched wall
+// grid fails it; the interface cells of a two-phase operator are few).
+int skip_gate(double cells, double aniso) { return cells > 0.0 && aniso * 10.0 <= cells; }
+
+void set_relax(ngap_t *h, int skip) {
+    for (int l = 0; l < h->nlev; l++) {
+        Level &L = h->lev[l];
+        L.relax = true;
+        if (!skip || l == 0 || l == h->nlev - 1) continue;
+        const int n[3] = {L.g.n0, L.g.n1, L.g.n2};
+        double smin = INFINITY, smax = 0.0;
+        int cnt = 0;
+        for (int d = 0; d < 3; d++) {
+            if (n[d] < 2 || !(L.str[d] > 0.0)) continue;
+            cnt++;
+            smin = std::min(smin, L.str[d]);
+            smax = std::max(smax, L.str[d]);
+        }
+        L.relax = cnt < 2 || smax <= 2.0 * (1.0 + 1e-6) * smin;
+    }
+}
+
 int can_coarsen(const Geo &g, int d) {
     int n = d == 0 ? g.n0 : (d == 1 ? g.n1 : g.n2);
     int per = d == 0 ? g.p0 : (d == 1 ? g.p1 : g.p2);
@@ -910,7 +947,7 @@ void launch_bottom_levels(ngap_t *h, int lb) {
     for (int q = 0; q < a.nlev; q++) {
         Level &L = h->lev[lb + q];
         LevelData<T> &D = data<T>(L);
-        a.L[q] = BotLevel<T>{L.g, L.cdir, D.c, D.a0, D.a1, D.a2, D.pm, D.pp, D.f, D.v, D.w};
+        a.L[q] = BotLevel<T>{L.g, L.cdir, L.relax ? 1 : 0, D.c, D.a0, D.a1, D.a2, D.pm, D.pp, D.f, D.v, D.w};
     }
     launch_bottom<T>(h->stream, a);
 }
@@ -947,6 +984,14 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         LevelData<T> &D = data<T>(L);
         const T *f = l == 0 ? f0 : D.f;
         T *fin = l == 0 ? v0 : D.v;
+        if (!L.relax) {
+            // skipped level: v = 0, so the residual is f; restrict it as it is
+            Level &Cn = h->lev[l + 1];
+            LevelData<T> &DC = data<T>(Cn);
+            launch_gather<T>(s, L.g, Cn.g, L.cdir, f, DC.pm, DC.pp, DC.f);
+            pre[l] = nullptr;
+            continue;
+        }
         const int npl = post_launches(L), prl = pre_launches(L);
         const bool pairs = L.g.N >= PAIR_MIN_CELLS;
         T *bufs[2] = {((prl + npl) & 1) ? fin : D.w, ((prl + npl) & 1) ? D.w : fin};
@@ -989,6 +1034,11 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
         T *fin = l == 0 ? v0 : D.v;
         Level &Cn = h->lev[l + 1];
         LevelData<T> &DC = data<T>(Cn);
+        if (!L.relax) {
+            // skipped level: v = P v_coarse into the level's final buffer
+            launch_prolong_to<T>(s, L.g, Cn.g, L.cdir, DC.pm, DC.pp, DC.v, (const T *)nullptr, fin);
+            continue;
+        }
         T *cur = pre[l];
         launch_prolong_to<T>(s, L.g, Cn.g, L.cdir, DC.pm, DC.pp, DC.v, cur, cur);
         for (int sw = 0; sw < npost;) {
@@ -1057,6 +1107,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     if (op.nu_pre < 1) op.nu_pre = op.precision == NGAP_MIXED ? 2 : 1;
     if (op.nu_post < 1) op.nu_post = op.precision == NGAP_MIXED ? 2 : 1;
     if (op.coarsest_max < 1) op.coarsest_max = 128;
+    if (op.skip_relax < 0 || op.skip_relax > 1) return NGAP_ERR_ARG;
     if (op.precision != NGAP_FP64 && op.precision != NGAP_MIXED) return NGAP_ERR_ARG;
     if (h->method == NGAP_GMRES_PFMG && (op.kdim < 1 || op.kdim > 30)) return NGAP_ERR_ARG;
     h->opts = op;
@@ -1138,6 +1189,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
         HIPCHK(hipGetLastError());
         if ((rc = read_scal(h))) return rc;
         double str[3];
+        for (int d = 0; d < 3; d++) F.str[d] = h->h_scal[S_STR0 + d];
         for (int d = 0; d < 3; d++) str[d] = can_coarsen(F.g, d) ? h->h_scal[S_STR0 + d] : -1.0;
         int best = -1;
         for (int d = 0; d < 3; d++) {
@@ -1173,6 +1225,23 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     for (size_t q = nlev; q < h->lev.size(); q++) free_level(h, h->lev[q]);
     h->lev.resize(nlev);
     h->nlev = nlev;
+    int skip = op.skip_relax;
+    if (skip && nlev > 2) {
+        // the skip applies only to a fine level that is locally isotropic
+        int np = 0;
+        Level &F0 = h->lev[0];
+        launch_aniso_count(s, F0.g, F0.d.a0, F0.d.a1, F0.d.a2, h->parts, &np);
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)h->parts, np, h->scal, (int)S_STR0);
+        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), np, h->scal,
+                           (int)S_STR1);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scal(h))) return rc;
+        skip = skip_gate(h->h_scal[S_STR0], h->h_scal[S_STR1]);
+    }
+    set_relax(h, skip);
+    h->st.skip_relax = skip;
+    h->st.relaxed_levels = 0;
+    for (int l = 0; l < nlev; l++) h->st.relaxed_levels += h->lev[l].relax ? 1 : 0;
     HIPCHK(hipStreamSynchronize(s));
     Level &C = h->lev[nlev - 1];
     h->coarse_dense = C.g.N <= 1024;
@@ -2050,6 +2119,7 @@ int ngap_default_opts(ngap_opts *o) {
     o->verbose = 0;
     o->graphs = 1;
     o->kdim = 5;
+    o->skip_relax = 1;
     return 0;
 }
 

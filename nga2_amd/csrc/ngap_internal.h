@@ -34,7 +34,7 @@ void launch_resid(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
 template <typename T>
 void launch_gather(hipStream_t s, const Geo &F, const Geo &C, int d, const T *r, const T *pm, const T *pp, T *fc,
                    bool vec = true);
-// vout = vin + P vc (vout may be vin)
+// vout = vin + P vc (vout may be vin; vin == nullptr: vout = P vc)
 template <typename T>
 void launch_prolong_to(hipStream_t s, const Geo &F, const Geo &C, int d, const T *pm, const T *pp, const T *vc,
                        const T *vin, T *vout, bool vec = true);
@@ -48,7 +48,8 @@ constexpr long long BOT_MAX_CELLS = 1024;
 template <typename T>
 struct BotLevel {
     Geo g;
-    int cdir;  // coarsening direction to the next level
+    int cdir;   // coarsening direction to the next level
+    int relax;  // 0: skipped level (no smoothing, see set_relax)
     const T *c, *a0, *a1, *a2;
     const T *pm, *pp;  // transfer weights from the finer level (coarse-indexed)
     T *f, *v, *w;
@@ -67,6 +68,10 @@ void launch_coarse_build(hipStream_t s, const Geo &F, const Geo &C, int d, const
                          double *ca1, double *ca2, double *sc);
 void launch_strength(hipStream_t s, const Geo &g, const double *a0, const double *a1, const double *a2, double *part,
                      int *nparts);
+// counts of the cells with two or more directions and of the anisotropic ones
+// (k_aniso_count) as block partials: part[0..nparts), part[MAX_PARTS..)
+void launch_aniso_count(hipStream_t s, const Geo &g, const double *a0, const double *a1, const double *a2,
+                        double *part, int *nparts);
 void launch_to_f32(hipStream_t s, long long n, const double *src, float *dst);
 void launch_to_f64(hipStream_t s, long long n, const float *src, double *dst);
 
@@ -82,6 +87,8 @@ struct LevelData {
 struct Level {
     Geo g{};
     int cdir = -1;
+    bool relax = true;              // smoothed in the V-cycle (skipped relaxation)
+    double str[3] = {0.0, 0.0, 0.0}; // sum |a_d| per direction, from the coarsening choice
     LevelData<double> d;
     LevelData<float> s;
 };

@@ -172,6 +172,44 @@ __global__ void __launch_bounds__(NT) k_strength(Geo g, const double *__restrict
     }
 }
 
+// Local anisotropy of a level: per cell with two or more directions of
+// n_d > 1, s_d = |a_d(p)| + |a_d(p + e_d)| (its two couplings along d); the
+// cell counts (part) when some s_d > 0 and anisotropic (part2) when
+// max_d s_d > 2 (1 + 1e-6) min_d s_d. Block partials as doubles (exact counts).
+__global__ void __launch_bounds__(NT) k_aniso_count(Geo g, const double *__restrict__ a0,
+                                                    const double *__restrict__ a1, const double *__restrict__ a2,
+                                                    double *__restrict__ part) {
+    __shared__ double sm[NT / 64];
+    const double *a[3] = {a0, a1, a2};
+    const int n[3] = {g.n0, g.n1, g.n2}, per[3] = {g.p0, g.p1, g.p2};
+    double cnt = 0.0, an = 0.0;
+    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
+        const int i = (int)(p % g.n0), j = (int)((p / g.n0) % g.n1), k = (int)(p / ((long long)g.n0 * g.n1));
+        const int c3[3] = {i, j, k};
+        const long long stride[3] = {1, g.n0, (long long)g.n0 * g.n1};
+        double mn = INFINITY, mx = 0.0;
+        int nd = 0;
+        for (int d = 0; d < 3; d++) {
+            if (n[d] < 2) continue;
+            double sd = fabs(a[d][p]);
+            const int q = nbp(c3[d], n[d], per[d]);
+            if (q >= 0) sd += fabs(a[d][p + (long long)(q - c3[d]) * stride[d]]);
+            nd++;
+            mn = fmin(mn, sd);
+            mx = fmax(mx, sd);
+        }
+        if (nd < 2 || !(mx > 0.0)) continue;
+        cnt += 1.0;
+        if (mx > 2.0 * (1.0 + 1e-6) * mn) an += 1.0;
+    }
+    const double bc = block_sum(cnt, sm);
+    const double ba = block_sum(an, sm);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = bc;
+        part[MAX_PARTS + blockIdx.x] = ba;
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_to_f32(long long n, const double *__restrict__ s, float *__restrict__ d) {
     for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < n; p += (long long)gridDim.x * NT) d[p] = (float)s[p];
 }
@@ -209,6 +247,13 @@ void launch_strength(hipStream_t s, const Geo &g, const double *a0, const double
     int b = grid_for(g.N);
     *nparts = b;
     hipLaunchKernelGGL(k_strength, dim3(b), dim3(NT), 0, s, g, a0, a1, a2, part);
+}
+
+void launch_aniso_count(hipStream_t s, const Geo &g, const double *a0, const double *a1, const double *a2,
+                        double *part, int *nparts) {
+    int b = grid_for(g.N);
+    *nparts = b;
+    hipLaunchKernelGGL(k_aniso_count, dim3(b), dim3(NT), 0, s, g, a0, a1, a2, part);
 }
 
 void launch_to_f32(hipStream_t s, long long n, const double *src, float *dst) {

@@ -840,7 +840,7 @@ __global__ void __launch_bounds__(256) k_prolong_t(Geo F, Geo C, const T *__rest
             add = fma_t(pp[pl], vc[pr], add);
         }
     }
-    vout[p] = vin[p] + add;
+    vout[p] = (vin ? vin[p] : T(0)) + add;
 }
 
 // The same prolongation for coarsening in y (D = 1) or z (D = 2) on levels
@@ -877,7 +877,13 @@ __global__ void __launch_bounds__(256) k_prolong_v(Geo F, Geo C, const T *__rest
         }
     }
     // vin may be vout: this thread reads its cells before it writes them
-    const V a = *reinterpret_cast<const V *>(vin + p);
+    V a;
+    if (vin) {
+        a = *reinterpret_cast<const V *>(vin + p);
+    } else {
+#pragma unroll
+        for (int q = 0; q < CPL; q++) a.v[q] = T(0);
+    }
     V o;
 #pragma unroll
     for (int q = 0; q < CPL; q++) o.v[q] = a.v[q] + add.v[q];
@@ -909,7 +915,13 @@ __global__ void __launch_bounds__(256) k_prolong_xv(Geo F, Geo C, const T *__res
     }
     const int jr = nbp(J0 + H - 1, C.n0, C.p0);
     cv[H] = jr >= 0 ? vc[rowc + jr] : T(0);
-    const V a = *reinterpret_cast<const V *>(vin + p);  // vin may be vout: read before the write below
+    V a;  // vin may be vout: read before the write below
+    if (vin) {
+        a = *reinterpret_cast<const V *>(vin + p);
+    } else {
+#pragma unroll
+        for (int q = 0; q < CPL; q++) a.v[q] = T(0);
+    }
     V o;
 #pragma unroll
     for (int q = 0; q < CPL; q++) {
@@ -1139,6 +1151,13 @@ __global__ void __launch_bounds__(BT_THREADS) k_bottom(BotArgs<T> a) {
     // down: pre-smoothing, residual, restriction
     for (int l = 0; l < nl - 1; l++) {
         const BotLevel<T> &L = a.L[l];
+        if (!L.relax) {
+            // skipped level: v = 0 (the prolongation adds to it), f restricted as it is
+            for (int p = threadIdx.x; p < (int)L.g.N; p += BT_THREADS) L.v[p] = T(0);
+            bt_gather<T>(L.g, a.L[l + 1].g, L.cdir, L.f, a.L[l + 1].pm, a.L[l + 1].pp, a.L[l + 1].f);
+            __syncthreads();
+            continue;
+        }
         for (int sw = 0; sw < a.npre; sw++) {
             bt_colour<T>(L, L.v, 0, sw == 0);
             __syncthreads();
@@ -1168,6 +1187,7 @@ __global__ void __launch_bounds__(BT_THREADS) k_bottom(BotArgs<T> a) {
         const BotLevel<T> &L = a.L[l];
         bt_prolong<T>(L.g, a.L[l + 1].g, L.cdir, a.L[l + 1].pm, a.L[l + 1].pp, a.L[l + 1].v, L.v);
         __syncthreads();
+        if (!L.relax) continue;
         for (int sw = 0; sw < a.npost; sw++) {
             bt_colour<T>(L, L.v, 1, false);
             __syncthreads();

@@ -34,11 +34,12 @@ module ngap_c
    end type ngap_grid_t
 
    type, bind(C), public :: ngap_opts_t
-      integer(c_int) :: precision,nu_pre,nu_post,coarsest_max,check_finite,verbose,graphs,kdim
+      integer(c_int) :: precision,nu_pre,nu_post,coarsest_max,check_finite,verbose,graphs,kdim,skip_relax
    end type ngap_opts_t
 
    type, bind(C), public :: ngap_stats_t
       integer(c_int) :: status,it,nlevels,coarse_singular,precision,fallback,restarts,stage_calls
+      integer(c_int) :: skip_relax,relaxed_levels
       real(c_double) :: rerr,true_rerr,aerr,cnorm_ratio,asym,setup_ms,solve_ms,copy_ms
       integer(c_long_long) :: cells,device_bytes
    end type ngap_stats_t

@@ -32,6 +32,10 @@ def lib():
         L.orc_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _dp,
                                  C.c_int, C.c_int, C.c_int, C.c_int]
         L.orc_create.restype = C.c_int
+        L.orc_set_skip_relax.argtypes = [C.c_void_p, C.c_int]
+        L.orc_set_skip_relax.restype = None
+        L.orc_relax.argtypes = [C.c_void_p, C.c_int]
+        L.orc_relax.restype = C.c_int
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_nlevels.argtypes = [C.c_void_p]
         L.orc_coarse_singular.argtypes = [C.c_void_p]
@@ -82,7 +86,7 @@ def to_field(flat, cfg, out=None):
 class Oracle:
     """FP64 CPU PCG-PFMG on the interior of one grid."""
 
-    def __init__(self, n3, per3, opr7, maxlevel=0, coarsest_max=128, nu_pre=1, nu_post=1):
+    def __init__(self, n3, per3, opr7, maxlevel=0, coarsest_max=128, nu_pre=1, nu_post=1, skip_relax=1):
         self.n3 = tuple(int(v) for v in n3)
         self.N = self.n3[0] * self.n3[1] * self.n3[2]
         op = np.ascontiguousarray(opr7, dtype=np.float64).reshape(self.N, 7)
@@ -93,6 +97,11 @@ class Oracle:
         if rc != 0:
             raise RuntimeError(f"orc_create failed: {rc}")
         self.h = h
+        lib().orc_set_skip_relax(h, int(skip_relax))
+
+    def relax_levels(self):
+        """Per level, 1 when the V-cycle smooths it (orc_set_skip_relax)."""
+        return [lib().orc_relax(self.h, l) for l in range(self.nlevels)]
 
     @classmethod
     def from_cfg(cls, cfg, opr, **kw):

@@ -90,6 +90,7 @@ typedef struct {
     double *chol; /* n x n lower factor, row-major */
     int nc;
     double *bufs[8];
+    unsigned char relax[ORC_MAXLEV]; /* 1: level l is smoothed in the V-cycle (see orc_set_skip_relax) */
 } orc_t;
 
 static inline long idx3(const int *n, int i, int j, int k) { return (long)i + (long)n[0] * ((long)j + (long)n[1] * k); }
@@ -501,7 +502,8 @@ static void vcycle(orc_t *o, const double *r, double *z) {
         const double *f = l == 0 ? f0 : L->f;
         double *v = l == 0 ? z : L->v;
         memset(v, 0, sizeof(double) * L->N);
-        for (int s = 0; s < o->nu_pre; s++) { gs_colour(L, f, v, 0); gs_colour(L, f, v, 1); }
+        if (o->relax[l])
+            for (int s = 0; s < o->nu_pre; s++) { gs_colour(L, f, v, 0); gs_colour(L, f, v, 1); }
         resid_restrict(o, l, f, v, o->L[l + 1].f);
     }
     {
@@ -515,7 +517,8 @@ static void vcycle(orc_t *o, const double *r, double *z) {
         const double *f = l == 0 ? f0 : L->f;
         double *v = l == 0 ? z : L->v;
         prolong_add(o, l, o->L[l + 1].v, v);
-        for (int s = 0; s < o->nu_post; s++) { gs_colour(L, f, v, 1); gs_colour(L, f, v, 0); }
+        if (o->relax[l])
+            for (int s = 0; s < o->nu_post; s++) { gs_colour(L, f, v, 1); gs_colour(L, f, v, 0); }
     }
 }
 
@@ -673,6 +676,7 @@ int orc_create(orc_t **out, int nx, int ny, int nz, int xper, int yper, int zper
         if (build_coarse(o, o->nlev - 1)) { orc_destroy(o); return -1; }
         o->nlev++;
     }
+    for (int l = 0; l < ORC_MAXLEV; l++) o->relax[l] = 1;
     lvl_t *C = &o->L[o->nlev - 1];
     o->coarse_dense = C->N <= 1024;
     if (o->coarse_dense) {
@@ -682,6 +686,74 @@ int orc_create(orc_t **out, int nx, int ny, int nz, int xper, int yper, int zper
     *out = o;
     return 0;
 }
+
+/* Skipped relaxation (PFMG's SkipRelax idea, rule of this project): with
+ * skip != 0 a level l with 0 < l < nlev-1 is smoothed only when its couplings
+ * are isotropic: over the directions with n_d > 1 and s_d = sum_p |a_d(p)| > 0,
+ * max s_d <= 2 min s_d (a level with fewer than two such directions counts as
+ * isotropic); the factor 2 carries a 1e-6 relative margin, so a ratio of 2 up
+ * to the rounding of the sums counts as isotropic whatever the summation order. On a skipped level the V-cycle restricts the right-hand side
+ * unchanged on the way down (v = 0) and sets v = P v_coarse on the way up.
+ * After semicoarsening x, y, z of an isotropic grid the strengths go
+ * (1,1,1) -> (1/2,2,2) -> (1,1,4) -> (2,2,2): every third level is smoothed,
+ * as in full coarsening. The finest level is always smoothed, the coarsest
+ * always solved. skip == 0 smooths every level. */
+/* The skip applies only to a locally isotropic fine operator: per cell with
+ * two or more directions of n_d > 1, s_d = |a_d(p)| + |a_d(p + e_d)|; a cell
+ * with some s_d > 0 counts, and is anisotropic when max_d s_d > 2 (1 + 1e-6)
+ * min_d s_d. The gate is open when at most 10% of the counted cells are
+ * anisotropic (a stretched wall grid fails it). */
+static int skip_gate(const lvl_t *L) {
+    double cnt = 0.0, an = 0.0;
+    for (int k = 0; k < L->n[2]; k++)
+        for (int j = 0; j < L->n[1]; j++)
+            for (int i = 0; i < L->n[0]; i++) {
+                long p = idx3(L->n, i, j, k);
+                int c3[3] = {i, j, k};
+                double mn = INFINITY, mx = 0.0;
+                int nd = 0;
+                for (int d = 0; d < 3; d++) {
+                    if (L->n[d] < 2) continue;
+                    double sd = fabs(L->a[d][p]);
+                    int q = nbp(c3[d], L->n[d], L->per[d]);
+                    if (q >= 0) {
+                        int e3[3] = {i, j, k};
+                        e3[d] = q;
+                        sd += fabs(L->a[d][idx3(L->n, e3[0], e3[1], e3[2])]);
+                    }
+                    nd++;
+                    if (sd < mn) mn = sd;
+                    if (sd > mx) mx = sd;
+                }
+                if (nd < 2 || !(mx > 0.0)) continue;
+                cnt += 1.0;
+                if (mx > 2.0 * (1.0 + 1e-6) * mn) an += 1.0;
+            }
+    return cnt > 0.0 && an * 10.0 <= cnt;
+}
+
+void orc_set_skip_relax(orc_t *o, int skip) {
+    if (skip && o->nlev > 2) skip = skip_gate(&o->L[0]);
+    for (int l = 0; l < o->nlev; l++) {
+        o->relax[l] = 1;
+        if (!skip || l == 0 || l == o->nlev - 1) continue;
+        const lvl_t *L = &o->L[l];
+        double smin = INFINITY, smax = 0.0;
+        int cnt = 0;
+        for (int d = 0; d < 3; d++) {
+            if (L->n[d] < 2) continue;
+            double sd = 0.0;
+            for (long p = 0; p < L->N; p++) sd += fabs(L->a[d][p]);
+            if (!(sd > 0.0)) continue;
+            cnt++;
+            if (sd < smin) smin = sd;
+            if (sd > smax) smax = sd;
+        }
+        o->relax[l] = cnt < 2 || smax <= 2.0 * (1.0 + 1e-6) * smin;
+    }
+}
+
+int orc_relax(const orc_t *o, int l) { return (l >= 0 && l < o->nlev) ? o->relax[l] : -1; }
 
 int orc_nlevels(const orc_t *o) { return o->nlev; }
 int orc_coarse_singular(const orc_t *o) { return o->coarse_singular; }

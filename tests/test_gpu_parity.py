@@ -248,3 +248,44 @@ def test_solve_matches_oracle(case, precision):
     P.cfg.sync(chk)
     assert np.array_equal(chk, ps.sol)
     ps.destroy()
+
+
+@pytest.mark.parametrize("name", list(CASES))
+def test_skip_relax_matches_oracle(name):
+    """Skipped relaxation (opts.skip_relax = 1): the GPU V-cycle in both
+    precisions against the oracle's with the same skip rule, and the solves
+    against the oracle's solution."""
+    print(f"\n[skip_relax {name}]", flush=True)
+    P = CASES[name]()
+    o = Oracle.from_cfg(P.cfg, P.opr, skip_relax=1)
+    ps = _solver(P, precision=MIXED, nu_pre=1, nu_post=1, skip_relax=1)
+    hk = ps.hooks()
+    rng = np.random.default_rng(3)
+    r = rng.standard_normal(o.N)
+    r -= r.mean()
+    zo = o.vcycle(r)
+    assert _close(hk.vcycle(r), zo, 1e-9)
+    assert _close(hk.vcycle(r, prec=1), zo, 1e-3)
+    # the skip changes the cycle wherever it skips a level
+    o_all = Oracle.from_cfg(P.cfg, P.opr, skip_relax=0)
+    if 0 in o.relax_levels():
+        assert not _close(o_all.vcycle(r), zo, 1e-6)
+    o_all.close()
+    ps.destroy()
+    b = interior(P.rhs, P.cfg)
+    xr, _ = o.solve(b, rcvg=1e-13, maxit=500)
+    _, st_10 = o.solve(b, rcvg=1e-10, maxit=500)
+    for precision in (0, MIXED):
+        ps = _solver(P, precision=precision, skip_relax=1)
+        ps.rhs[:] = P.rhs
+        ps.sol[:] = 0.0
+        ps.solve()
+        st = ps.stats()
+        x = interior(ps.sol, P.cfg)
+        print(f"  prec={precision} it={ps.it} oracle_it={st_10['it']} relax={o.relax_levels()}", flush=True)
+        assert st["status"] == 0 and st["true_rerr"] <= 1e-10
+        assert rel_err_nullspace(x, xr, P.cfg, P.opr) <= 1e-8
+        if precision == 0:
+            assert abs(ps.it - st_10["it"]) <= 1
+        ps.destroy()
+    o.close()

@@ -271,3 +271,64 @@ def test_odd_periodic_rejected():
     opr = incomp.laplacian(incomp.Metrics(cfg))
     with pytest.raises(RuntimeError):
         Oracle.from_cfg(cfg, opr)
+
+
+def test_skip_relax_rule_and_cycle():
+    """Skipped relaxation: on the uniform periodic cube semicoarsening x, y, z
+    takes the direction strengths (1,1,1) -> (1/2,2,2) -> (1,1,4) -> (2,2,2), so
+    exactly every third level is smoothed (and the coarsest is solved). With
+    the skip the V-cycle stays symmetric positive definite, equals the cycle
+    that smooths every level when nothing is skipped (a two-level hierarchy),
+    and still contracts as a stationary iteration on the two-phase problem."""
+    P = inputs.periodic_gaussian(n=32)
+    o = Oracle.from_cfg(P.cfg, P.opr, skip_relax=1)
+    rl = o.relax_levels()
+    nl = o.nlevels
+    assert rl == [1 if (l % 3 == 0 or l == nl - 1) else 0 for l in range(nl)], rl
+    # the strengths behind the rule, from the level coefficients
+    for l in range(1, nl - 1):
+        c = o.level_coefs(l)
+        s = sorted(np.abs(c[a]).sum() for a in ("ax", "ay", "az"))
+        assert (s[2] <= 2 * (1 + 1e-6) * s[0]) == bool(rl[l]), (l, s)
+    rng = np.random.default_rng(5)
+    u, v = rng.standard_normal(o.N), rng.standard_normal(o.N)
+    u -= u.mean()
+    v -= v.mean()
+    Mu, Mv = o.vcycle(u), o.vcycle(v)
+    assert abs(Mu @ v - u @ Mv) <= 1e-12 * np.linalg.norm(Mu) * np.linalg.norm(v)
+    assert u @ Mu > 0 and v @ Mv > 0
+    # two levels: nothing can be skipped, the cycles agree bit for bit
+    o2s = Oracle.from_cfg(P.cfg, P.opr, maxlevel=2, skip_relax=1)
+    o2 = Oracle.from_cfg(P.cfg, P.opr, maxlevel=2)
+    assert np.array_equal(o2s.vcycle(u), o2.vcycle(u))
+    # contraction with the skip on the density-ratio problem
+    Ps = inputs.spheres_two_phase(n=16, nsph=4, rmin=0.15, rmax=0.3)
+    os_ = Oracle.from_cfg(Ps.cfg, Ps.opr, skip_relax=1)
+    b = interior(Ps.rhs, Ps.cfg)
+    xs, _ = os_.solve(b, rcvg=1e-14, maxit=300)
+    x = np.zeros_like(b)
+    errs = []
+    for _ in range(6):
+        x = x + os_.vcycle(b - os_.apply(0, x))
+        e = x - xs
+        e -= e.mean()
+        errs.append(np.sqrt(e @ os_.apply(0, e)))
+    assert max(errs[i + 1] / errs[i] for i in range(5)) < 0.95
+
+
+def test_skip_relax_gate():
+    """The skip applies only to a locally isotropic fine operator: the
+    tanh-stretched channel (wall cells with y couplings many times the x ones)
+    smooths every level; the two-phase spheres (a few interface cells) and
+    the solid spheres (one-sided couplings at the walls, ratio 2) skip."""
+    Pc = inputs.channel_spectral(n=32)
+    oc = Oracle.from_cfg(Pc.cfg, Pc.opr, skip_relax=1)
+    assert all(r == 1 for r in oc.relax_levels())
+    for P in (inputs.spheres_two_phase(n=32, nsph=6, rmin=0.1, rmax=0.2),
+              inputs.masked_spheres(n=32, nsph=4, rmin=0.1, rmax=0.2)):
+        o = Oracle.from_cfg(P.cfg, P.opr, skip_relax=1)
+        assert 0 in o.relax_levels()
+        assert o.relax_levels()[0] == 1 and o.relax_levels()[-1] == 1
+    # skip_relax=0 smooths every level
+    o0 = Oracle.from_cfg(P.cfg, P.opr, skip_relax=0)
+    assert all(r == 1 for r in o0.relax_levels())
