@@ -64,40 +64,37 @@ def test_stage_shape_mismatch_is_an_error(seq_solver):
 
 
 def test_learned_guess_beats_previous_step_guess(seq_solver):
-    """Trained on 48 solves of the sequence, the learned guess starts the next
-    8 solves from a far smaller residual than the previous solution does, and
-    they need fewer iterations.
+    """Configuration 4 as specified: trained on the solves of steps 0-149, the
+    learned guess starts steps 150-157 from a far smaller residual than the
+    previous solution does, and the network removes an iteration.
 
-    - The temporal predictor (m = 6) is fitted on the solutions.
+    - The temporal predictor (m = 8) is fitted on the solutions.
     - The network stage is trained with the residual loss on the residuals of
       the extrapolated guesses.
-    - The network's correction lowers the residual further, or is rejected by
-      the stage."""
-    print("\n[learned guess: train 48 steps, evaluate 8]", flush=True)
+    - The network's correction lowers the residual the predictor leaves on
+      every step, and the solve from it needs fewer iterations than the solve
+      from the predictor alone."""
+    print("\n[learned guess: train 150 steps, evaluate 8]", flush=True)
     S = seq_solver
-    sols, st = learned.collect(S, range(48))
-    alpha = learned.fit_predictor(sols, 6)
-    pairs = learned.stage_pairs(S, sols, alpha, range(7, 48))
+    sols, st = learned.collect(S, range(150))
+    alpha = learned.fit_predictor(sols, 8)
+    pairs = learned.stage_pairs(S, sols, alpha, range(9, 150))
     kappa = learned.fit_kappa(pairs)
     net = UNet((128, 128, 128), seed=0)
-    hist = learned.train(net, pairs, kappa, S=S, epochs=4, lr=5e-4, log=lambda m: print("  " + m, flush=True))
+    hist = learned.train(net, pairs, kappa, S=S, epochs=6, lr=5e-4, log=lambda m: print("  " + m, flush=True))
     del pairs
-    recs = learned.evaluate(S, net, alpha, sols, range(48, 56))
+    recs = learned.evaluate(S, net, alpha, sols, range(150, 158))
     s = learned.summary(recs)
     print("  " + ", ".join(f"{k} {v:.3g}" if isinstance(v, float) else f"{k} {v}" for k, v in s.items()), flush=True)
     assert min(hist) < hist[0]
     assert max(s["max_true_prev"], s["max_true_extrap"], s["max_true_learned"]) <= RCVG
     assert s["max_solution_diff"] <= 1e-8
     assert s["r0_extrap"] < 1e-2 * s["r0_prev"]
-    # What the network gains over the method without it: its correction lowers
-    # the residual the temporal predictor leaves, on every evaluated step. At
-    # 128^3 the predictor leaves ~1e-6 and the solve then needs 3 iterations;
-    # one iteration fewer would need a ~6x lower start, and the network gives
-    # ~2x, so the iteration count is required not to rise.
-    assert s["r0_learned"] < s["r0_extrap"]
+    # what the network gains over the method without it: a lower starting
+    # residual on every step, and fewer iterations
     assert all(r["r0_learned"] < r["r0_extrap"] for r in recs)
     assert s["it_extrap"] <= s["it_prev"] - 3
-    assert s["it_learned"] <= s["it_extrap"]
+    assert s["it_learned"] < s["it_extrap"]
     net.close()
 
 
