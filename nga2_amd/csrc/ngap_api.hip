@@ -623,6 +623,7 @@ struct ngap_t {
     double *h_gm = nullptr;  // pinned copy
     int fail_alloc = 0;      // > 0: the fail_alloc-th next device allocation fails (ngapt_fail_alloc)
     Staging stg;             // pinned staging of ngap_solve, made on its first call
+    int relax_keep = 1;      // levels 0..relax_keep-1 are always smoothed (set_relax)
     long long zero_chunks = 0;  // guess chunks of the last ngap_solve found all zero (not copied)
 };
 
@@ -778,19 +779,21 @@ int read_scal(ngap_t *h) {
 // (1,1,4) -> (2,2,2), so every third level is smoothed, as in full
 // coarsening. A skipped level restricts its right-hand side unchanged on the
 // way down (v = 0) and takes v = P v_coarse on the way up. The finest level is
-// always smoothed and the coarsest always solved.
-// Skipped relaxation is for isotropic operators: PFMG's semicoarsening
-// handles anisotropy by coarsening the strong direction, and smoothing only
-// every few levels loses that. The gate opens when at most 10% of the fine
-// level's cells are anisotropic by k_aniso_count's measure (a stretched wall
-// grid fails it; the interface cells of a two-phase operator are few).
+// always smoothed, the coarsest always solved, and levels below relax_keep
+// smoothed (all of them on a locally anisotropic operator).
+// Skipping the intermediate levels suits isotropic operators. A locally
+// anisotropic fine operator (more than 10% of its cells anisotropic by
+// k_aniso_count's measure: a stretched wall grid) smooths every level:
+// skipping took the channel from 13 to 26 iterations, and keeping the first
+// coarsening round smoothed still to 17 at 256^3. The interface cells of a
+// two-phase operator are few, and its gate stays open.
 int skip_gate(double cells, double aniso) { return cells > 0.0 && aniso * 10.0 <= cells; }
 
 void set_relax(ngap_t *h, int skip) {
     for (int l = 0; l < h->nlev; l++) {
         Level &L = h->lev[l];
         L.relax = true;
-        if (!skip || l == 0 || l == h->nlev - 1) continue;
+        if (!skip || l < h->relax_keep || l == h->nlev - 1) continue;
         const int n[3] = {L.g.n0, L.g.n1, L.g.n2};
         double smin = INFINITY, smax = 0.0;
         int cnt = 0;
@@ -1226,6 +1229,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     h->lev.resize(nlev);
     h->nlev = nlev;
     int skip = op.skip_relax;
+    h->relax_keep = 1;
     if (skip && nlev > 2) {
         // the skip applies only to a fine level that is locally isotropic
         int np = 0;
@@ -1236,7 +1240,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
                            (int)S_STR1);
         HIPCHK(hipGetLastError());
         if ((rc = read_scal(h))) return rc;
-        skip = skip_gate(h->h_scal[S_STR0], h->h_scal[S_STR1]);
+        h->relax_keep = skip_gate(h->h_scal[S_STR0], h->h_scal[S_STR1]) ? 1 : nlev;
     }
     set_relax(h, skip);
     h->st.skip_relax = skip;

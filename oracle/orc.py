@@ -36,6 +36,8 @@ def lib():
         L.orc_set_skip_relax.restype = None
         L.orc_relax.argtypes = [C.c_void_p, C.c_int]
         L.orc_relax.restype = C.c_int
+        L.orc_set_relax_level.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.orc_set_relax_level.restype = C.c_int
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_nlevels.argtypes = [C.c_void_p]
         L.orc_coarse_singular.argtypes = [C.c_void_p]
@@ -98,6 +100,11 @@ class Oracle:
             raise RuntimeError(f"orc_create failed: {rc}")
         self.h = h
         lib().orc_set_skip_relax(h, int(skip_relax))
+
+    def set_relax_levels(self, flags):
+        """Set every level's smoothing flag (a test hook for other skip patterns)."""
+        for l, f in enumerate(flags):
+            lib().orc_set_relax_level(self.h, l, int(f))
 
     def relax_levels(self):
         """Per level, 1 when the V-cycle smooths it (orc_set_skip_relax)."""

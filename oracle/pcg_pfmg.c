@@ -697,12 +697,13 @@ int orc_create(orc_t **out, int nx, int ny, int nz, int xper, int yper, int zper
  * After semicoarsening x, y, z of an isotropic grid the strengths go
  * (1,1,1) -> (1/2,2,2) -> (1,1,4) -> (2,2,2): every third level is smoothed,
  * as in full coarsening. The finest level is always smoothed, the coarsest
- * always solved. skip == 0 smooths every level. */
+ * always solved, and a locally anisotropic fine operator (skip_gate) smooths
+ * every level. skip == 0 smooths every level. */
 /* The skip applies only to a locally isotropic fine operator: per cell with
  * two or more directions of n_d > 1, s_d = |a_d(p)| + |a_d(p + e_d)|; a cell
  * with some s_d > 0 counts, and is anisotropic when max_d s_d > 2 (1 + 1e-6)
  * min_d s_d. The gate is open when at most 10% of the counted cells are
- * anisotropic (a stretched wall grid fails it). */
+ * anisotropic (a stretched wall grid fails it and smooths every level). */
 static int skip_gate(const lvl_t *L) {
     double cnt = 0.0, an = 0.0;
     for (int k = 0; k < L->n[2]; k++)
@@ -733,10 +734,12 @@ static int skip_gate(const lvl_t *L) {
 }
 
 void orc_set_skip_relax(orc_t *o, int skip) {
-    if (skip && o->nlev > 2) skip = skip_gate(&o->L[0]);
+    /* a locally anisotropic fine operator (gate closed) smooths every level */
+    int keep = 1;
+    if (skip && o->nlev > 2 && !skip_gate(&o->L[0])) keep = o->nlev;
     for (int l = 0; l < o->nlev; l++) {
         o->relax[l] = 1;
-        if (!skip || l == 0 || l == o->nlev - 1) continue;
+        if (!skip || l < keep || l == o->nlev - 1) continue;
         const lvl_t *L = &o->L[l];
         double smin = INFINITY, smax = 0.0;
         int cnt = 0;
@@ -754,6 +757,13 @@ void orc_set_skip_relax(orc_t *o, int skip) {
 }
 
 int orc_relax(const orc_t *o, int l) { return (l >= 0 && l < o->nlev) ? o->relax[l] : -1; }
+
+/* set one level's flag directly (test hook: cycles with other skip patterns) */
+int orc_set_relax_level(orc_t *o, int l, int relax) {
+    if (l < 0 || l >= o->nlev) return -2;
+    o->relax[l] = relax != 0;
+    return 0;
+}
 
 int orc_nlevels(const orc_t *o) { return o->nlev; }
 int orc_coarse_singular(const orc_t *o) { return o->coarse_singular; }
