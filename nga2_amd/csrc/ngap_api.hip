@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -22,7 +23,8 @@ using namespace ngap;
 namespace {
 
 // ---------------------------------------------------------------- scalar slots
-enum { S_RR = 0, S_BB, S_RHO, S_ALPHA, S_BETA, S_PQ, S_RHO0, S_BAD, S_RRT, S_ASYM, S_STR0, S_STR1, S_STR2, S_NSCAL };
+enum { S_RR = 0, S_BB, S_RHO, S_ALPHA, S_BETA, S_PQ, S_RHO0, S_BAD, S_RRT, S_ASYM, S_STR0, S_STR1, S_STR2, S_CSING,
+       S_CBAD, S_NSCAL };
 constexpr int NSCAL_ALLOC = 16;
 // device and pinned host scalars, then 4 int flags in the same allocation
 constexpr size_t SCAL_BYTES = sizeof(double) * NSCAL_ALLOC + sizeof(int) * 4;
@@ -56,52 +58,89 @@ __device__ __forceinline__ long long lin0(const Geo &g, int i, int j, int k) {
 
 // Level-0 symmetric storage from the ghosted AoS opr (hypre_str_class.f90:
 // 171-179: rows with opr(1)==0 become identity rows).
+//
+// The 7 entries of a cell are adjacent in opr, so a block's consecutive cells
+// own one contiguous run per grid row. The block first copies its cells'
+// entries through LDS with coalesced 8-byte loads (and, with opr_copy, writes
+// them to the solver's own copy in the same coalesced order); each thread then
+// takes its cell's entries from LDS. The minus neighbours' centre and plus
+// coupling are read from memory.
+constexpr int OPR_NST = 7;  // init accepts 7-point stencils only
 __global__ void __launch_bounds__(NT) k_level0(Geo g, Ghost gh, Slots sl, const double *__restrict__ opr,
-                                               double *__restrict__ c, double *__restrict__ a0,
-                                               double *__restrict__ a1, double *__restrict__ a2,
+                                               double *__restrict__ opr_copy, double *__restrict__ c,
+                                               double *__restrict__ a0, double *__restrict__ a1,
+                                               double *__restrict__ a2, LevelData<float> s32,
                                                unsigned char *__restrict__ part, double *__restrict__ p_asym,
                                                double *__restrict__ p_bad) {
     __shared__ double sm[NT / 64];
+    __shared__ double st[NT * OPR_NST];
+    __shared__ int sg[NT];
     double asym = 0.0, bad = 0.0;
-    double *a[3] = {a0, a1, a2};
     const int n3[3] = {g.n0, g.n1, g.n2}, per3[3] = {g.p0, g.p1, g.p2};
-    for (long long p = blockIdx.x * (long long)NT + threadIdx.x; p < g.N; p += (long long)gridDim.x * NT) {
-        int c3[3];
-        decomp0(g, p, c3[0], c3[1], c3[2]);
-        const double *A = opr + gidx(gh, c3[0], c3[1], c3[2]) * sl.nst;
+    const int N = (int)g.N;  // cell indices fit in int (setup rejects 2^31 cells)
+    // the loop is uniform over the block (barriers inside)
+    for (int pb = blockIdx.x * NT; pb < N; pb += gridDim.x * NT) {
+        const int p = pb + threadIdx.x;
+        const bool ok = p < N;
+        int c3[3] = {0, 0, 0};
+        if (ok) {
+            const unsigned rq = (unsigned)p / (unsigned)g.n0;
+            c3[0] = p - (int)rq * g.n0;
+            c3[1] = (int)(rq % (unsigned)g.n1);
+            c3[2] = (int)(rq / (unsigned)g.n1);
+        }
+        __syncthreads();  // the previous cells' entries are no longer read
+        sg[threadIdx.x] = ok ? (int)gidx(gh, c3[0], c3[1], c3[2]) : -1;
+        __syncthreads();
+        for (int e = threadIdx.x; e < NT * OPR_NST; e += NT) {
+            const int cl = e / OPR_NST, q = e - cl * OPR_NST;
+            const int gs = sg[cl];
+            if (gs >= 0) {
+                const double val = opr[(long long)gs * OPR_NST + q];
+                st[e] = val;
+                if (opr_copy) opr_copy[(long long)gs * OPR_NST + q] = val;
+            }
+        }
+        __syncthreads();
+        if (!ok) continue;
         double v[7];
         for (int s = 0; s < 7; s++) {
-            v[s] = A[sl.s[s]];
+            v[s] = st[threadIdx.x * OPR_NST + sl.s[s]];
             if (!isfinite(v[s])) bad += 1.0;
         }
         const bool pt = fabs(v[0]) > 0.0;
         part[p] = pt ? 1 : 0;
         double cc = pt ? v[0] : 1.0;
+        double av[3];
+#pragma unroll
         for (int d = 0; d < 3; d++) {
+            av[d] = 0.0;
             double am = v[2 + 2 * d], ap = v[1 + 2 * d];
             if (n3[d] == 1) {
                 if (per3[d] && pt) cc += am + ap;
-                a[d][p] = 0.0;
                 continue;
             }
             int q = nbm(c3[d], n3[d], per3[d]);
-            if (q < 0 || !pt) {
-                a[d][p] = 0.0;
-                continue;
-            }
+            if (q < 0 || !pt) continue;
             int e3[3] = {c3[0], c3[1], c3[2]};
             e3[d] = q;
-            const double *B = opr + gidx(gh, e3[0], e3[1], e3[2]) * sl.nst;
-            if (!(fabs(B[sl.s[0]]) > 0.0)) {
-                a[d][p] = 0.0;
-                continue;
-            }
+            const double *B = opr + gidx(gh, e3[0], e3[1], e3[2]) * OPR_NST;
+            if (!(fabs(B[sl.s[0]]) > 0.0)) continue;
             double apn = B[sl.s[1 + 2 * d]];
-            a[d][p] = 0.5 * (am + apn);
+            av[d] = 0.5 * (am + apn);
             double rel = fabs(am - apn) / fabs(v[0]);
             if (rel > asym) asym = rel;
         }
         c[p] = cc;
+        a0[p] = av[0];
+        a1[p] = av[1];
+        a2[p] = av[2];
+        if (s32.c) {
+            s32.c[p] = (float)cc;
+            s32.a0[p] = (float)av[0];
+            s32.a1[p] = (float)av[1];
+            s32.a2[p] = (float)av[2];
+        }
     }
     double m = block_max(asym, sm);
     double b = block_sum(bad, sm);
@@ -491,6 +530,16 @@ __global__ void __launch_bounds__(NT) k_fin_sum(const double *__restrict__ parts
     if (threadIdx.x == 0) scal[slot] = s;
 }
 
+// scal[slot + v] = sum of parts[v * MAX_PARTS + 0..n-1], v < nv, each in k_fin_sum's order
+__global__ void __launch_bounds__(NT) k_fin_sums(const double *__restrict__ parts, int n, int nv,
+                                                 double *__restrict__ scal, int slot) {
+    __shared__ double sm[NT / 64];
+    for (int v = 0; v < nv; v++) {
+        double s = sum_parts(parts + (size_t)v * MAX_PARTS, n, sm);
+        if (threadIdx.x == 0) scal[slot + v] = s;
+    }
+}
+
 __global__ void __launch_bounds__(NT) k_fin_max(const double *__restrict__ parts, int n, double *__restrict__ scal,
                                                 int slot) {
     __shared__ double sm[NT / 64];
@@ -606,6 +655,7 @@ struct ngap_t {
     int coarse_dense = 0, coarse_singular = 0;
     int apply_parts = 0;  // partial sums written by the fine-level SpMV
     hipGraphExec_t vc_graph[2] = {nullptr, nullptr};  // V-cycle per precision (0 FP64, 1 FP32)
+    std::vector<long long> vc_key[2];                  // what each graph recorded (cycle_key)
     ngap_stats st{};
     long long dev_bytes = 0;
     std::vector<void *> base_allocs;
@@ -625,6 +675,10 @@ struct ngap_t {
     Staging stg;             // pinned staging of ngap_solve, made on its first call
     int relax_keep = 1;      // levels 0..relax_keep-1 are always smoothed (set_relax)
     long long zero_chunks = 0;  // guess chunks of the last ngap_solve found all zero (not copied)
+    // the last setup completed with these options; its levels and coarsening
+    // directions are then the first guess of the next setup's hierarchy
+    bool spec_valid = false;
+    int spec_maxlevel = 0, spec_coarsest = 0;
 };
 
 namespace {
@@ -814,18 +868,61 @@ int can_coarsen(const Geo &g, int d) {
     return n >= 2;
 }
 
+// Coarsening direction of a level from its strengths sv[d] = sum |a_d|: the
+// strongest direction that can be coarsened (ties within a 1e-6 relative
+// margin to the lowest d), -1 when none can.
+int pick_dir(const Geo &g, const double *sv) {
+    double str[3];
+    for (int d = 0; d < 3; d++) str[d] = can_coarsen(g, d) ? sv[d] : -1.0;
+    int best = -1;
+    for (int d = 0; d < 3; d++) {
+        if (str[d] < 0) continue;
+        if (best < 0 || str[d] > str[best] * (1.0 + 1e-6)) best = d;
+    }
+    return best;
+}
+
 // ---------------------------------------------------------------- coarsest (host)
 
-// Dense coarsest operator -> (pseudo-)inverse, FP64, on the host.
+// sum of a[k] b[k] over k in [lo, hi), in four interleaved partial sums
+// (the compiler keeps a single running sum in order and cannot vectorise it)
+double dot4(const double *a, const double *b, int lo, int hi) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int k = lo;
+    for (; k + 4 <= hi; k += 4) {
+        s0 += a[k] * b[k];
+        s1 += a[k + 1] * b[k + 1];
+        s2 += a[k + 2] * b[k + 2];
+        s3 += a[k + 3] * b[k + 3];
+    }
+    for (; k < hi; k++) s0 += a[k] * b[k];
+    return (s0 + s1) + (s2 + s3);
+}
+
+// the coarsest inverse's buffer for n cells
+int ensure_ainv(ngap_t *h, int n) {
+    if (h->Ainv && h->nc != n) {
+        dfree(h, h->Ainv, sizeof(double) * (size_t)h->nc * h->nc);
+        h->Ainv = nullptr;
+    }
+    h->nc = n;
+    if (!h->Ainv && dmalloc(h, (void **)&h->Ainv, sizeof(double) * (size_t)n * n)) return NGAP_ERR_ALLOC;
+    return 0;
+}
+
+// Dense coarsest operator -> (pseudo-)inverse, FP64, on the host (coarsest
+// levels of more than COARSE_LDS_MAX cells; k_coarse_inverse below that).
 int build_coarsest_inverse(ngap_t *h) {
+    int rc;
     Level &C = h->lev[h->nlev - 1];
     const Geo &g = C.g;
     int n = (int)g.N;
     std::vector<double> c(n), a0(n), a1(n), a2(n);
-    HIPCHK(hipMemcpy(c.data(), C.d.c, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(a0.data(), C.d.a0, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(a1.data(), C.d.a1, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(a2.data(), C.d.a2, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(c.data(), C.d.c, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(a0.data(), C.d.a0, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(a1.data(), C.d.a1, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(a2.data(), C.d.a2, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
     const double *a[3] = {a0.data(), a1.data(), a2.data()};
     const int n3[3] = {g.n0, g.n1, g.n2}, p3[3] = {g.p0, g.p1, g.p2};
     std::vector<double> M((size_t)n * n, 0.0);
@@ -863,41 +960,34 @@ int build_coarsest_inverse(ngap_t *h) {
         double gam = csum / ((double)n * n);
         for (auto &m : M) m += gam;
     }
-    // Cholesky (lower)
+    // Cholesky (lower), then A^-1 = W^T W with W = L^-1. Rows are contiguous,
+    // so every inner product runs over contiguous memory (dot4).
     std::vector<double> L((size_t)n * n, 0.0);
     for (int j = 0; j < n; j++) {
-        double s = M[(size_t)j * n + j];
-        for (int k = 0; k < j; k++) s -= L[(size_t)j * n + k] * L[(size_t)j * n + k];
+        const double *Lj = &L[(size_t)j * n];
+        double s = M[(size_t)j * n + j] - dot4(Lj, Lj, 0, j);
         if (!(s > 0.0)) return NGAP_ERR_COARSE;
         double ljj = std::sqrt(s);
         L[(size_t)j * n + j] = ljj;
-        for (int i = j + 1; i < n; i++) {
-            double t = M[(size_t)i * n + j];
-            for (int k = 0; k < j; k++) t -= L[(size_t)i * n + k] * L[(size_t)j * n + k];
-            L[(size_t)i * n + j] = t / ljj;
-        }
+        for (int i = j + 1; i < n; i++)
+            L[(size_t)i * n + j] = (M[(size_t)i * n + j] - dot4(&L[(size_t)i * n], Lj, 0, j)) / ljj;
     }
-    // inverse column by column (symmetric result)
-    std::vector<double> X((size_t)n * n, 0.0), y(n);
+    // Y[col][i] = W[i][col] (zero for i < col): forward substitution on e_col
+    std::vector<double> Y((size_t)n * n, 0.0), X((size_t)n * n, 0.0);
     for (int col = 0; col < n; col++) {
-        for (int i = 0; i < n; i++) {
-            double s = (i == col) ? 1.0 : 0.0;
-            for (int k = 0; k < i; k++) s -= L[(size_t)i * n + k] * y[k];
-            y[i] = s / L[(size_t)i * n + i];
+        double *y = &Y[(size_t)col * n];
+        for (int i = col; i < n; i++) {
+            const double *Li = &L[(size_t)i * n];
+            y[i] = ((i == col ? 1.0 : 0.0) - dot4(Li, y, col, i)) / Li[i];
         }
-        for (int i = n - 1; i >= 0; i--) {
-            double s = y[i];
-            for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * y[k];
-            y[i] = s / L[(size_t)i * n + i];
+    }
+    for (int a = 0; a < n; a++)
+        for (int b = a; b < n; b++) {
+            const double v = dot4(&Y[(size_t)a * n], &Y[(size_t)b * n], b, n);
+            X[(size_t)a * n + b] = v;
+            X[(size_t)b * n + a] = v;
         }
-        for (int i = 0; i < n; i++) X[(size_t)col * n + i] = y[i];
-    }
-    if (h->Ainv && h->nc != n) {
-        dfree(h, h->Ainv, sizeof(double) * (size_t)h->nc * h->nc);
-        h->Ainv = nullptr;
-    }
-    h->nc = n;
-    if (!h->Ainv && dmalloc(h, (void **)&h->Ainv, sizeof(double) * (size_t)n * n)) return NGAP_ERR_ALLOC;
+    if ((rc = ensure_ainv(h, n))) return rc;
     HIPCHK(hipMemcpy(h->Ainv, X.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice));
     return 0;
 }
@@ -1060,11 +1150,49 @@ void vcycle(ngap_t *h, const T *f0, T *v0) {
 
 // ---------------------------------------------------------------- graphs
 
+void destroy_graph(ngap_t *h, int prec) {
+    if (h->vc_graph[prec]) (void)hipGraphExecDestroy(h->vc_graph[prec]);
+    h->vc_graph[prec] = nullptr;
+    h->vc_key[prec].clear();
+}
+
 void destroy_graphs(ngap_t *h) {
-    for (auto &g : h->vc_graph) {
-        if (g) (void)hipGraphExecDestroy(g);
-        g = nullptr;
+    destroy_graph(h, 0);
+    destroy_graph(h, 1);
+}
+
+// Everything the recorded V-cycle of one precision depends on: the launch
+// sequence follows from the level shapes, coarsening directions, relaxed
+// levels, sweep counts and the coarsest solve; the kernel arguments are these
+// and the buffer addresses. A setup that leaves all of them as they were
+// (a new operator on the same grid and hierarchy) replays the graph it has.
+std::vector<long long> cycle_key(const ngap_t *h, int prec) {
+    std::vector<long long> k;
+    auto P = [&k](const void *p) { k.push_back((long long)(uintptr_t)p); };
+    k.insert(k.end(), {(long long)prec, (long long)h->nlev, (long long)h->opts.nu_pre, (long long)h->opts.nu_post,
+                       (long long)h->coarse_dense, (long long)h->nc});
+    P(h->Ainv);
+    P(prec ? (const void *)h->r32 : (const void *)h->r);
+    P(prec ? (const void *)h->z32 : (const void *)h->z);
+    for (int l = 0; l < h->nlev; l++) {
+        const Level &L = h->lev[l];
+        k.insert(k.end(), {(long long)L.g.n0, (long long)L.g.n1, (long long)L.g.n2, (long long)L.g.p0,
+                           (long long)L.g.p1, (long long)L.g.p2, (long long)L.cdir, (long long)L.relax});
+        if (prec) {
+            const LevelData<float> &D = L.s;
+            for (const void *q : {(const void *)D.c, (const void *)D.a0, (const void *)D.a1, (const void *)D.a2,
+                                  (const void *)D.pm, (const void *)D.pp, (const void *)D.f, (const void *)D.v,
+                                  (const void *)D.w})
+                P(q);
+        } else {
+            const LevelData<double> &D = L.d;
+            for (const void *q : {(const void *)D.c, (const void *)D.a0, (const void *)D.a1, (const void *)D.a2,
+                                  (const void *)D.pm, (const void *)D.pp, (const void *)D.f, (const void *)D.v,
+                                  (const void *)D.w})
+                P(q);
+        }
     }
+    return k;
 }
 
 // Record the V-cycle on the handle's stream (f0 = r / r32, v0 = z / z32).
@@ -1087,6 +1215,7 @@ int capture_vcycle(ngap_t *h, int prec) {
         h->vc_graph[prec] = nullptr;
         return NGAP_ERR_HIP;
     }
+    h->vc_key[prec] = cycle_key(h, prec);
     return 0;
 }
 
@@ -1118,8 +1247,9 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     h->rcvg = rcvg;
     h->maxlevel = maxlevel;
     HIPCHK(hipSetDevice(h->device));
-    // tile kernels index with int32 and write one partial per SpMV block
-    if (h->g0.N >= (1LL << 31)) return NGAP_ERR_UNSUPPORTED;
+    // tile kernels index with int32 and write one partial per SpMV block;
+    // k_level0 indexes the ghosted cells with int32
+    if (h->g0.N >= (1LL << 31) || h->nghost >= (1LL << 31)) return NGAP_ERR_UNSUPPORTED;
     {
         int kc;
         if (apply_blocks(h->g0, kc) > MAX_PARTS) return NGAP_ERR_UNSUPPORTED;
@@ -1150,12 +1280,14 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     hipStream_t s = h->stream;
     size_t obytes = sizeof(double) * (size_t)h->nghost * h->nst;
     if (on_device) {
-        // the caller's producer may run on another stream: wait for the device first
+        // the caller's producer may run on another stream: wait for the device
+        // first; k_level0 then reads the caller's array and writes the
+        // solver's copy of its interior in the same pass
         HIPCHK(hipDeviceSynchronize());
-        HIPCHK(hipMemcpyAsync(h->opr, opr, obytes, hipMemcpyDeviceToDevice, s));
     } else {
         HIPCHK(hipMemcpyAsync(h->opr, opr, obytes, hipMemcpyHostToDevice, s));
     }
+    const bool mixed = op.precision == NGAP_MIXED;
     // level 0
     if (h->lev.empty()) {
         h->lev.resize(1);
@@ -1164,20 +1296,71 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
             return NGAP_ERR_ALLOC;
     }
     Level &L0 = h->lev[0];
-    int nb = grid_for(h->g0.N);
+    int nb = grid_parts(h->g0.N);
+    // level 0 in FP64, and in FP32 for the mixed-precision cycle
+    hipLaunchKernelGGL(k_level0, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots,
+                       on_device ? opr : (const double *)h->opr, on_device ? h->opr : (double *)nullptr, L0.d.c,
+                       L0.d.a0, L0.d.a1, L0.d.a2, mixed ? L0.s : LevelData<float>{}, h->part, h->parts,
+                       h->parts + MAX_PARTS);
     if (h->method == NGAP_GMRES_PFMG)
         hipLaunchKernelGGL(k_soa7, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots, (const double *)h->opr, h->o7);
-    hipLaunchKernelGGL(k_level0, dim3(nb), dim3(NT), 0, s, h->g0, h->gh, h->slots, (const double *)h->opr, L0.d.c,
-                       L0.d.a0, L0.d.a1, L0.d.a2, h->part, h->parts, h->parts + MAX_PARTS);
     hipLaunchKernelGGL(k_fin_max, dim3(1), dim3(NT), 0, s, (const double *)h->parts, nb, h->scal, (int)S_ASYM);
     hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
                        (int)S_BAD);
     HIPCHK(hipGetLastError());
-    if ((rc = read_scal(h))) return rc;
-    h->st.asym = h->h_scal[S_ASYM];
-    if (op.check_finite && h->h_scal[S_BAD] > 0.0) return NGAP_ERR_NONFINITE;
-    // coarser levels
+    // Coarser levels. A setup on a handle whose last setup built a hierarchy
+    // with the same options first rebuilds that hierarchy with its coarsening
+    // directions, without waiting for the strengths; one read at the end
+    // brings every level's strengths, and the directions are checked against
+    // them. Levels from the first direction that differs are built again by
+    // the loop below, which reads each level's strengths before coarsening it.
+    const int nprev = (int)h->lev.size();
+    const bool spec = h->spec_valid && nprev >= 2 && h->spec_maxlevel == maxlevel &&
+                      h->spec_coarsest == op.coarsest_max;
+    h->spec_valid = false;
     int nlev = 1;
+    bool have_aniso = false;
+    double aniso[2] = {0.0, 0.0};
+    if (spec) {
+        double *lstr = h->parts + 3 * MAX_PARTS;  // unused by the other setup reductions
+        for (int l = 0; l + 1 < nprev; l++) {
+            Level &F = h->lev[l];
+            Level &C = h->lev[l + 1];
+            int np = 0;
+            launch_strength(s, F.g, F.d.a0, F.d.a1, F.d.a2, h->parts, &np);
+            hipLaunchKernelGGL(k_fin_sums, dim3(1), dim3(NT), 0, s, (const double *)h->parts, np, 3, lstr, 3 * l);
+            launch_coarse_build(s, F.g, C.g, F.cdir, F.d.c, F.d.a0, F.d.a1, F.d.a2, C.d.pm, C.d.pp, C.d.c, C.d.a0,
+                                C.d.a1, C.d.a2, C.d.f, mixed ? &C.s : nullptr);
+        }
+        if (op.skip_relax) {
+            int np = 0;
+            launch_aniso_count(s, L0.g, L0.d.a0, L0.d.a1, L0.d.a2, h->parts, &np);
+            hipLaunchKernelGGL(k_fin_sums, dim3(1), dim3(NT), 0, s, (const double *)h->parts, np, 2, lstr,
+                               3 * (nprev - 1));
+        }
+        HIPCHK(hipGetLastError());
+        std::vector<double> hs(3 * (size_t)nprev);
+        HIPCHK(hipMemcpyAsync(hs.data(), lstr, sizeof(double) * hs.size(), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipMemcpyAsync(h->h_scal, h->scal, SCAL_BYTES, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        h->st.asym = h->h_scal[S_ASYM];
+        if (op.check_finite && h->h_scal[S_BAD] > 0.0) return NGAP_ERR_NONFINITE;
+        for (int l = 0; l + 1 < nprev; l++) {
+            Level &F = h->lev[l];
+            for (int d = 0; d < 3; d++) F.str[d] = hs[3 * l + d];
+            if (pick_dir(F.g, &hs[3 * l]) != F.cdir) break;
+            nlev = l + 2;
+        }
+        if (op.skip_relax) {
+            have_aniso = true;
+            aniso[0] = hs[3 * (nprev - 1)];
+            aniso[1] = hs[3 * (nprev - 1) + 1];
+        }
+    } else {
+        if ((rc = read_scal(h))) return rc;
+        h->st.asym = h->h_scal[S_ASYM];
+        if (op.check_finite && h->h_scal[S_BAD] > 0.0) return NGAP_ERR_NONFINITE;
+    }
     while (true) {
         Level &F = h->lev[nlev - 1];
         F.cdir = -1;
@@ -1186,19 +1369,12 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
         if (nlev >= 64) break;
         int np = 0;
         launch_strength(s, F.g, F.d.a0, F.d.a1, F.d.a2, h->parts, &np);
-        for (int d = 0; d < 3; d++)
-            hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + d * MAX_PARTS), np,
-                               h->scal, (int)(S_STR0 + d));
+        hipLaunchKernelGGL(k_fin_sums, dim3(1), dim3(NT), 0, s, (const double *)h->parts, np, 3, h->scal,
+                           (int)S_STR0);
         HIPCHK(hipGetLastError());
         if ((rc = read_scal(h))) return rc;
-        double str[3];
         for (int d = 0; d < 3; d++) F.str[d] = h->h_scal[S_STR0 + d];
-        for (int d = 0; d < 3; d++) str[d] = can_coarsen(F.g, d) ? h->h_scal[S_STR0 + d] : -1.0;
-        int best = -1;
-        for (int d = 0; d < 3; d++) {
-            if (str[d] < 0) continue;
-            if (best < 0 || str[d] > str[best] * (1.0 + 1e-6)) best = d;
-        }
+        const int best = pick_dir(F.g, F.str);
         if (best < 0) break;
         Geo cg = F.g;
         int per = best == 0 ? F.g.p0 : (best == 1 ? F.g.p1 : F.g.p2);
@@ -1221,7 +1397,7 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
         C.g = cg;
         C.cdir = -1;
         launch_coarse_build(s, Fr.g, cg, best, Fr.d.c, Fr.d.a0, Fr.d.a1, Fr.d.a2, C.d.pm, C.d.pp, C.d.c, C.d.a0,
-                            C.d.a1, C.d.a2, C.d.f);
+                            C.d.a1, C.d.a2, C.d.f, mixed ? &C.s : nullptr);
         HIPCHK(hipGetLastError());
         nlev++;
     }
@@ -1232,54 +1408,58 @@ int do_setup(ngap_t *h, const double *opr, bool on_device, int maxit, double rcv
     h->relax_keep = 1;
     if (skip && nlev > 2) {
         // the skip applies only to a fine level that is locally isotropic
-        int np = 0;
-        Level &F0 = h->lev[0];
-        launch_aniso_count(s, F0.g, F0.d.a0, F0.d.a1, F0.d.a2, h->parts, &np);
-        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)h->parts, np, h->scal, (int)S_STR0);
-        hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), np, h->scal,
-                           (int)S_STR1);
-        HIPCHK(hipGetLastError());
-        if ((rc = read_scal(h))) return rc;
-        h->relax_keep = skip_gate(h->h_scal[S_STR0], h->h_scal[S_STR1]) ? 1 : nlev;
+        if (!have_aniso) {
+            int np = 0;
+            Level &F0 = h->lev[0];
+            launch_aniso_count(s, F0.g, F0.d.a0, F0.d.a1, F0.d.a2, h->parts, &np);
+            hipLaunchKernelGGL(k_fin_sums, dim3(1), dim3(NT), 0, s, (const double *)h->parts, np, 2, h->scal,
+                               (int)S_STR0);
+            HIPCHK(hipGetLastError());
+            if ((rc = read_scal(h))) return rc;
+            aniso[0] = h->h_scal[S_STR0];
+            aniso[1] = h->h_scal[S_STR1];
+        }
+        h->relax_keep = skip_gate(aniso[0], aniso[1]) ? 1 : nlev;
     }
     set_relax(h, skip);
     h->st.skip_relax = skip;
     h->st.relaxed_levels = 0;
     for (int l = 0; l < nlev; l++) h->st.relaxed_levels += h->lev[l].relax ? 1 : 0;
-    HIPCHK(hipStreamSynchronize(s));
     Level &C = h->lev[nlev - 1];
     h->coarse_dense = C.g.N <= 1024;
     h->coarse_singular = 0;
-    if (h->coarse_dense) {
+    if (h->coarse_dense && C.g.N <= COARSE_LDS_MAX) {
+        if ((rc = ensure_ainv(h, (int)C.g.N))) return rc;
+        launch_coarse_inverse(s, C.g, C.d.c, C.d.a0, C.d.a1, C.d.a2, h->Ainv, h->scal + S_CSING);
+        HIPCHK(hipGetLastError());
+        if ((rc = read_scal(h))) return rc;
+        if (h->h_scal[S_CBAD] > 0.5) return NGAP_ERR_COARSE;
+        h->coarse_singular = h->h_scal[S_CSING] > 0.5;
+    } else if (h->coarse_dense) {
         if ((rc = build_coarsest_inverse(h))) return rc;
     }
-    // FP32 copies of the hierarchy for the mixed-precision V-cycle
-    if (op.precision == NGAP_MIXED) {
-        for (int l = 0; l < nlev; l++) {
-            Level &L = h->lev[l];
-            launch_to_f32(s, L.g.N, L.d.c, L.s.c);
-            launch_to_f32(s, L.g.N, L.d.a0, L.s.a0);
-            launch_to_f32(s, L.g.N, L.d.a1, L.s.a1);
-            launch_to_f32(s, L.g.N, L.d.a2, L.s.a2);
-            if (l > 0) {
-                launch_to_f32(s, L.g.N, L.d.pm, L.s.pm);
-                launch_to_f32(s, L.g.N, L.d.pp, L.s.pp);
-            }
-        }
-    }
+    // (the FP32 copies of the hierarchy for the mixed-precision V-cycle were
+    // written by k_level0 and the coarse build)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(s));
     {
         int kc;
         h->apply_parts = apply_blocks(h->lev[0].g, kc);
     }
-    // the V-cycle launch sequence is fixed once the hierarchy exists: replay it as a graph
-    destroy_graphs(h);
-    if (op.graphs) {
-        if ((rc = capture_vcycle(h, 0))) return rc;
-        if (op.precision == NGAP_MIXED && (rc = capture_vcycle(h, 1))) return rc;
+    // The V-cycle launch sequence is fixed once the hierarchy exists: replay it
+    // as a graph, recorded again only when what it recorded has changed. The
+    // mixed-precision solve records the FP32 cycle only; its FP64 fallback
+    // launches the FP64 cycle directly (or replays a graph still valid).
+    for (int prec = 0; prec < 2; prec++) {
+        const bool want = op.graphs && (prec == 1 ? mixed : !mixed);
+        const bool valid = h->vc_graph[prec] && h->vc_key[prec] == cycle_key(h, prec);
+        if (!op.graphs || !valid) destroy_graph(h, prec);
+        if (want && !valid && (rc = capture_vcycle(h, prec))) return rc;
     }
     h->setup_done = true;
+    h->spec_valid = true;
+    h->spec_maxlevel = maxlevel;
+    h->spec_coarsest = op.coarsest_max;
     h->st.nlevels = nlev;
     h->st.coarse_singular = h->coarse_singular;
     h->st.setup_ms = now_ms() - t0;

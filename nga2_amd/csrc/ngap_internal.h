@@ -10,6 +10,7 @@
 namespace ngap {
 
 int grid_for(long long n);
+int grid_parts(long long n);
 
 template <typename T>
 void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *x, T *y,
@@ -63,15 +64,18 @@ struct BotArgs {
 };
 template <typename T>
 void launch_bottom(hipStream_t s, const BotArgs<T> &a);
-void launch_coarse_build(hipStream_t s, const Geo &F, const Geo &C, int d, const double *c, const double *a0,
-                         const double *a1, const double *a2, double *pm, double *pp, double *cc, double *ca0,
-                         double *ca1, double *ca2, double *sc);
 void launch_strength(hipStream_t s, const Geo &g, const double *a0, const double *a1, const double *a2, double *part,
                      int *nparts);
 // counts of the cells with two or more directions and of the anisotropic ones
 // (k_aniso_count) as block partials: part[0..nparts), part[MAX_PARTS..)
 void launch_aniso_count(hipStream_t s, const Geo &g, const double *a0, const double *a1, const double *a2,
                         double *part, int *nparts);
+// The coarsest level's dense (pseudo-)inverse in one workgroup (levels of at
+// most COARSE_LDS_MAX cells): out[0] = 1 when singular, out[1] = 1 when the
+// Cholesky factorisation failed.
+constexpr int COARSE_LDS_MAX = 128;
+void launch_coarse_inverse(hipStream_t s, const Geo &g, const double *c, const double *a0, const double *a1,
+                           const double *a2, double *Ainv, double *out);
 void launch_to_f32(hipStream_t s, long long n, const double *src, float *dst);
 void launch_to_f64(hipStream_t s, long long n, const float *src, double *dst);
 
@@ -83,6 +87,12 @@ struct LevelData {
     T *f = nullptr, *v = nullptr;   // right-hand side and final iterate (levels >= 1)
     T *w = nullptr;                 // second iterate buffer (out-of-place sweeps), all levels
 };
+
+// The coarse level's weights (pm, pp), couplings and diagonal from the fine
+// level along direction d; with c32, their FP32 copies in the same pass.
+void launch_coarse_build(hipStream_t s, const Geo &F, const Geo &C, int d, const double *c, const double *a0,
+                         const double *a1, const double *a2, double *pm, double *pp, double *cc, double *ca0,
+                         double *ca1, double *ca2, double *sc, const LevelData<float> *c32);
 
 struct Level {
     Geo g{};

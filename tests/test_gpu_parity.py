@@ -289,3 +289,106 @@ def test_skip_relax_matches_oracle(name):
             assert abs(ps.it - st_10["it"]) <= 1
         ps.destroy()
     o.close()
+
+
+def _scaled(P, d, f):
+    """P's operator with its couplings along direction d scaled by f and the
+    centre moved so that every row sum stays (identity rows stay identity)."""
+    opr = P.opr.copy()
+    s_p, s_m = 1 + 2 * d, 2 + 2 * d
+    act = opr[0] != 0.0
+    opr[0] = np.where(act, opr[0] - (f - 1.0) * (opr[s_p] + opr[s_m]), 0.0)
+    opr[s_p] = np.where(act, f * opr[s_p], opr[s_p])
+    opr[s_m] = np.where(act, f * opr[s_m], opr[s_m])
+    return opr
+
+
+@pytest.mark.parametrize("name", ["channel16", "rect40x6x18", "masked_block16x11x8", "rows128x64x128"])
+@pytest.mark.parametrize("precision", [0, MIXED])
+def test_setup_again_matches_fresh_setup(name, precision):
+    """A setup on a handle that has been set up before first rebuilds the
+    previous hierarchy with its coarsening directions and checks them against
+    the new strengths afterwards. Through a sequence of operators whose
+    strongest direction changes, each setup gives the hierarchy (shapes,
+    directions, coefficients) and the solve of a fresh handle, bit for bit."""
+    print(f"\n[setup again: {name} prec={precision}]", flush=True)
+    import torch
+
+    from nga2_amd.linsol import DeviceSolve
+
+    P = CASES[name]()
+    seq = [P.opr, _scaled(P, 0, 30.0), _scaled(P, 0, 30.0), _scaled(P, 2, 30.0), P.opr]
+    ps = _solver(P, precision=precision)
+    # the same sequence through the device entry points (k_level0 also
+    # writes the solver's copy of the operator there)
+    dev = torch.device("cuda:0")
+    to_dev = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to(dev)
+    ds = DeviceSolve(P.cfg)
+    ds.setup(to_dev(seq[0]), maxit=200, rcvg=1e-10, precision=precision)
+    d_rhs = to_dev(P.rhs)
+    d_sol = torch.zeros_like(d_rhs)
+    dirs = []
+    for q, opr in enumerate(seq[1:], 1):
+        ps.opr[:] = opr
+        ps.setup()
+        ds.setup(to_dev(opr), maxit=200, rcvg=1e-10, precision=precision)
+        fresh = NgapStr(P.cfg, "Pressure", PCG_PFMG, 7)
+        fresh.stc[:] = STC7
+        fresh.maxit, fresh.rcvg, fresh.precision = 200, 1e-10, precision
+        fresh.opr[:] = opr
+        fresh.init()
+        fresh.setup()
+        ha, hb, hd = ps.hooks(), fresh.hooks(), ds.hooks()
+        assert ha.nlevels == hb.nlevels == hd.nlevels, q
+        info = [ha.level_info(l) for l in range(ha.nlevels)]
+        assert info == [hb.level_info(l) for l in range(hb.nlevels)], q
+        assert info == [hd.level_info(l) for l in range(hd.nlevels)], q
+        dirs.append(tuple(i[1] for i in info))
+        for l in range(ha.nlevels):
+            ca, cb, cd = ha.level_coefs(l), hb.level_coefs(l), hd.level_coefs(l)
+            for k in ca:
+                assert np.array_equal(ca[k], cb[k]) and np.array_equal(cd[k], cb[k]), (q, l, k)
+        for s_ in (ps, fresh):
+            s_.rhs[:] = P.rhs
+            s_.sol[:] = 0.0
+            s_.solve()
+        d_sol.zero_()
+        it_d, _ = ds.solve(d_rhs, d_sol)
+        assert ps.it == fresh.it and np.array_equal(ps.sol, fresh.sol), q
+        x_d = d_sol.cpu().numpy().reshape(P.rhs.shape, order="F")
+        assert it_d == fresh.it and np.array_equal(interior(x_d, P.cfg), interior(fresh.sol, P.cfg)), q
+        assert ps.stats()["true_rerr"] <= 1e-10 and ds.stats()["true_rerr"] <= 1e-10
+        fresh.destroy()
+    print(f"  coarsening directions per setup: {dirs}", flush=True)
+    # the sequence does change the hierarchy's directions
+    assert len(set(dirs)) >= 2
+    ps.destroy()
+
+
+@pytest.mark.parametrize("name", ["periodic16", "channel16", "walls13x10x7", "masked_spheres16"])
+@pytest.mark.parametrize("coarsest", [16, 128, 512, 1024])
+def test_coarsest_inverse_matches_oracle(name, coarsest):
+    """The coarsest level's dense (pseudo-)inverse: built on the GPU in one
+    workgroup for levels of <= 128 cells, on the host above that. Both apply
+    like the oracle's coarse solve, singular (periodic) levels included, and
+    the full solve still meets the oracle."""
+    P = CASES[name]()
+    o = Oracle.from_cfg(P.cfg, P.opr, coarsest_max=coarsest)
+    ps = _solver(P, coarsest_max=coarsest)
+    hk = ps.hooks()
+    assert hk.nlevels == o.nlevels
+    n = o.level_size(o.nlevels - 1)
+    print(f"\n[coarsest {name} max {coarsest}] coarsest level {n} cells, singular {o.coarse_singular}", flush=True)
+    rng = np.random.default_rng(coarsest)
+    for _ in range(3):
+        fc = rng.standard_normal(n)
+        if o.coarse_singular:
+            fc -= fc.mean()
+        assert _close(hk.coarse_solve(fc), o.coarse_solve(fc), 1e-9)
+    assert ps.stats()["coarse_singular"] == int(o.coarse_singular)
+    ps.rhs[:] = P.rhs
+    ps.sol[:] = 0.0
+    ps.solve()
+    assert ps.stats()["status"] == 0 and ps.stats()["true_rerr"] <= 1e-10
+    ps.destroy()
+    o.close()
