@@ -161,7 +161,9 @@ int ngap_setup(ngap_t *h, const double *opr, int maxit, double rcvg, int maxleve
 /* Same as ngap_setup with opr already in device memory (same layout).
  * No counterpart in the reference: for callers that assemble the operator
  * on the GPU. The call synchronizes the device before reading d_opr, so the
- * producer may run on any stream. */
+ * producer may run on any stream. It reads the interior cells' entries once
+ * (ghost entries are not read) and keeps its own copy of them, so d_opr may
+ * change after the call. */
 int ngap_setup_device(ngap_t *h, const double *d_opr, int maxit, double rcvg, int maxlevel, const ngap_opts *o);
 
 /* Replaces hypre_str_solve (:322-386). sol holds the initial guess on entry
