@@ -44,6 +44,11 @@ int nthreads_for(long long n) {
     return (int)std::max<long long>(1, std::min<long long>(cap, n / 8192));
 }
 
+// threads for row-parallel work over a matrix of n rows and nnz nonzeros: the
+// coarse levels have few rows but long ones (about 100 nonzeros per row three
+// levels down), so their work follows the nonzeros
+int nthreads_for(long long n, long long nnz) { return nthreads_for(std::max(n, nnz / 8)); }
+
 // An exception in a worker (bad_alloc from a row's scratch vectors) is kept
 // per thread and rethrown on the caller's thread after every started thread
 // has been joined; a thread that cannot be started (system_error) stops the
@@ -110,7 +115,7 @@ double diag_of(const Csr &A, int i) {
 // S[q]: row i strongly depends on A.col[q]
 std::vector<char> strength(const Csr &A, double theta, double max_row_sum) {
     std::vector<char> S(A.col.size(), 0);
-    par_rows(A.n, nthreads_for(A.n), [&](int lo, int hi, int) {
+    par_rows(A.n, nthreads_for(A.n, (long long)A.col.size()), [&](int lo, int hi, int) {
     for (int i = lo; i < hi; i++) {
         const int a = A.rp[i], b = A.rp[i + 1];
         const double diag = diag_of(A, i);
@@ -150,27 +155,44 @@ struct Graph {
     std::vector<int> drp, dep, irp, inf;
 };
 
+// The dependencies row by row in parallel; the influences (the transpose)
+// are counted and placed with atomic increments, then each list is sorted,
+// which gives the ascending row order of a serial fill.
 Graph strong_graph(const Csr &A, const std::vector<char> &S) {
     Graph G;
-    G.drp.assign(A.n + 1, 0);
-    G.irp.assign(A.n + 1, 0);
-    for (int i = 0; i < A.n; i++)
-        for (int q = A.rp[i]; q < A.rp[i + 1]; q++)
-            if (S[q]) {
-                G.drp[i + 1]++;
-                G.irp[A.col[q] + 1]++;
-            }
+    const int n = A.n;
+    const int nt = nthreads_for(n, (long long)A.col.size());
+    G.drp.assign(n + 1, 0);
+    G.irp.assign(n + 1, 0);
+    par_rows(n, nt, [&](int lo, int hi, int) {
+        for (int i = lo; i < hi; i++) {
+            int cnt = 0;
+            for (int q = A.rp[i]; q < A.rp[i + 1]; q++)
+                if (S[q]) {
+                    cnt++;
+                    __atomic_fetch_add(&G.irp[A.col[q] + 1], 1, __ATOMIC_RELAXED);
+                }
+            G.drp[i + 1] = cnt;
+        }
+    });
     scan_counts(G.drp);
     scan_counts(G.irp);
-    G.dep.resize(G.drp[A.n]);
-    G.inf.resize(G.irp[A.n]);
-    std::vector<int> pd(G.drp.begin(), G.drp.end() - 1), pi(G.irp.begin(), G.irp.end() - 1);
-    for (int i = 0; i < A.n; i++)
-        for (int q = A.rp[i]; q < A.rp[i + 1]; q++)
-            if (S[q]) {
-                G.dep[pd[i]++] = A.col[q];
-                G.inf[pi[A.col[q]]++] = i;
-            }
+    G.dep.resize(G.drp[n]);
+    G.inf.resize(G.irp[n]);
+    std::vector<int> pi(G.irp.begin(), G.irp.end() - 1);
+    par_rows(n, nt, [&](int lo, int hi, int) {
+        for (int i = lo; i < hi; i++) {
+            int pd = G.drp[i];
+            for (int q = A.rp[i]; q < A.rp[i + 1]; q++)
+                if (S[q]) {
+                    G.dep[pd++] = A.col[q];
+                    G.inf[__atomic_fetch_add(&pi[A.col[q]], 1, __ATOMIC_RELAXED)] = i;
+                }
+        }
+    });
+    par_rows(n, nt, [&](int lo, int hi, int) {
+        for (int j = lo; j < hi; j++) std::sort(G.inf.begin() + G.irp[j], G.inf.begin() + G.irp[j + 1]);
+    });
     return G;
 }
 
@@ -231,49 +253,58 @@ std::vector<int> pmis(const Csr &A, const Graph &G) {
     return cf;
 }
 
+// scratch of truncate_row, one per thread (no allocation per row)
+struct TruncScratch {
+    std::vector<int> key, keep, c2;
+    std::vector<double> v2;
+};
+
 // keep the pmax largest |w| (ties: smaller coarse column first), rescale, sort by column
-void truncate_row(std::vector<int> &cols, std::vector<double> &vals, int pmax) {
+void truncate_row(std::vector<int> &cols, std::vector<double> &vals, int pmax, TruncScratch &w) {
     const int m = (int)cols.size();
     if (pmax > 0 && m > pmax) {
-        std::vector<int> key(m);
-        for (int q = 0; q < m; q++) key[q] = q;
-        std::sort(key.begin(), key.end(), [&](int x, int y) {
+        w.key.resize(m);
+        for (int q = 0; q < m; q++) w.key[q] = q;
+        std::sort(w.key.begin(), w.key.end(), [&](int x, int y) {
             const double ax = std::fabs(vals[x]), ay = std::fabs(vals[y]);
             if (ax != ay) return ax > ay;
             return cols[x] < cols[y];
         });
-        std::vector<int> keep(key.begin(), key.begin() + pmax);
-        std::sort(keep.begin(), keep.end());
+        w.keep.assign(w.key.begin(), w.key.begin() + pmax);
+        std::sort(w.keep.begin(), w.keep.end());
         double sp = 0.0, sn = 0.0, kp = 0.0, kn = 0.0;
         for (int q = 0; q < m; q++) {
             if (vals[q] > 0) sp += vals[q];
             if (vals[q] < 0) sn += vals[q];
         }
-        for (int q : keep) {
+        for (int q : w.keep) {
             if (vals[q] > 0) kp += vals[q];
             if (vals[q] < 0) kn += vals[q];
         }
         const double fp = kp != 0.0 ? sp / kp : 1.0, fn = kn != 0.0 ? sn / kn : 1.0;
-        std::vector<int> c2;
-        std::vector<double> v2;
-        for (int q : keep) {
-            c2.push_back(cols[q]);
-            v2.push_back(vals[q] * (vals[q] > 0 ? fp : fn));
+        w.c2.clear();
+        w.v2.clear();
+        for (int q : w.keep) {
+            w.c2.push_back(cols[q]);
+            w.v2.push_back(vals[q] * (vals[q] > 0 ? fp : fn));
         }
-        cols.swap(c2);
-        vals.swap(v2);
+        cols.swap(w.c2);
+        vals.swap(w.v2);
     }
-    std::vector<int> ord(cols.size());
-    for (size_t q = 0; q < ord.size(); q++) ord[q] = (int)q;
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cols[x] < cols[y]; });
-    std::vector<int> c3(cols.size());
-    std::vector<double> v3(cols.size());
-    for (size_t q = 0; q < ord.size(); q++) {
-        c3[q] = cols[ord[q]];
-        v3[q] = vals[ord[q]];
+    // stable insertion sort by column (rows are short; no temporary buffer)
+    const int k = (int)cols.size();
+    for (int u = 1; u < k; u++) {
+        const int c = cols[u];
+        const double v = vals[u];
+        int q = u - 1;
+        while (q >= 0 && cols[q] > c) {
+            cols[q + 1] = cols[q];
+            vals[q + 1] = vals[q];
+            q--;
+        }
+        cols[q + 1] = c;
+        vals[q + 1] = v;
     }
-    cols.swap(c3);
-    vals.swap(v3);
 }
 
 // rows [lo, hi) of P appended to (col, val), row lengths to len
@@ -285,6 +316,7 @@ void interp_rows(const Csr &A, const Graph &G, const std::vector<int> &cf, const
     std::vector<int> chat_pos(n, 0), sstamp(n, -1);
     std::vector<int> chat, cols;
     std::vector<double> wv, vals;
+    TruncScratch tw;
     for (int i = lo; i < hi; i++) {
         const size_t before = pcol.size();
         if (cf[i] == C_PT) {
@@ -356,7 +388,7 @@ void interp_rows(const Csr &A, const Graph &G, const std::vector<int> &cf, const
                 cols[q] = cmap[chat[q]];
                 vals[q] = -wv[q] / diag;
             }
-            truncate_row(cols, vals, pmax);
+            truncate_row(cols, vals, pmax, tw);
             pcol.insert(pcol.end(), cols.begin(), cols.end());
             pval.insert(pval.end(), vals.begin(), vals.end());
         }
@@ -373,7 +405,7 @@ Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int p
         if (cf[i] == C_PT) cmap[i] = nc++;
     std::vector<double> adiag(n);
     for (int i = 0; i < n; i++) adiag[i] = diag_of(A, i);
-    const int nt = nthreads_for(n);
+    const int nt = nthreads_for(n, (long long)A.col.size());
     std::vector<std::vector<int>> pcol(nt), len(nt);
     std::vector<std::vector<double>> pval(nt);
     par_rows(n, nt, [&](int lo, int hi, int t) {
@@ -396,22 +428,47 @@ Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int p
 
 }  // namespace
 
+// Rows of the transpose counted and filled with atomic increments in
+// parallel, then each row sorted by column: the order of a serial fill.
 Csr transpose(const Csr &A) {
     Csr T;
     T.n = A.ncol;
     T.ncol = A.n;
     T.rp.assign(T.n + 1, 0);
-    for (int c : A.col) T.rp[c + 1]++;
+    const int nt = nthreads_for(A.n, (long long)A.col.size());
+    par_rows(A.n, nt, [&](int lo, int hi, int) {
+        for (int q = A.rp[lo]; q < A.rp[hi]; q++) __atomic_fetch_add(&T.rp[A.col[q] + 1], 1, __ATOMIC_RELAXED);
+    });
     scan_counts(T.rp);
     T.col.resize(A.col.size());
     T.val.resize(A.col.size());
     std::vector<int> pos(T.rp.begin(), T.rp.end() - 1);
-    for (int i = 0; i < A.n; i++)
-        for (int q = A.rp[i]; q < A.rp[i + 1]; q++) {
-            const int p = pos[A.col[q]]++;
-            T.col[p] = i;
-            T.val[p] = A.val[q];
+    par_rows(A.n, nt, [&](int lo, int hi, int) {
+        for (int i = lo; i < hi; i++)
+            for (int q = A.rp[i]; q < A.rp[i + 1]; q++) {
+                const int p = __atomic_fetch_add(&pos[A.col[q]], 1, __ATOMIC_RELAXED);
+                T.col[p] = i;
+                T.val[p] = A.val[q];
+            }
+    });
+    par_rows(T.n, nthreads_for(T.n, (long long)T.col.size()), [&](int lo, int hi, int) {
+        for (int r = lo; r < hi; r++) {
+            // insertion sort of the row's (column, value) pairs: rows are short
+            const int a = T.rp[r], b = T.rp[r + 1];
+            for (int u = a + 1; u < b; u++) {
+                const int c = T.col[u];
+                const double v = T.val[u];
+                int w = u - 1;
+                while (w >= a && T.col[w] > c) {
+                    T.col[w + 1] = T.col[w];
+                    T.val[w + 1] = T.val[w];
+                    w--;
+                }
+                T.col[w + 1] = c;
+                T.val[w + 1] = v;
+            }
         }
+    });
     return T;
 }
 
@@ -491,7 +548,7 @@ struct RowAcc {
 }  // namespace
 
 Csr spgemm(const Csr &A, const Csr &B) {
-    const int nt = nthreads_for(A.n);
+    const int nt = nthreads_for(A.n, (long long)A.col.size() + (long long)B.col.size());
     Csr C;
     C.n = A.n;
     C.ncol = B.ncol;
@@ -599,7 +656,7 @@ int build(Hier &H, Csr A, const ngap_uns_opts &o) {
     H.d.clear();
     for (const Csr &L : H.A) {
         std::vector<double> d(L.n);
-        par_rows(L.n, nthreads_for(L.n), [&](int lo, int hi, int) {
+        par_rows(L.n, nthreads_for(L.n, (long long)L.col.size()), [&](int lo, int hi, int) {
             for (int i = lo; i < hi; i++) {
                 double s = 0.0;
                 for (int q = L.rp[i]; q < L.rp[i + 1]; q++) s += std::fabs(L.val[q]);
