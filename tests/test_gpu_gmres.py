@@ -150,3 +150,39 @@ def test_full_size_channel():
         xs.append(d_x.cpu().numpy())
         ds.destroy()
     assert np.linalg.norm(xs[0] - xs[1]) <= 1e-8 * np.linalg.norm(xs[0])
+
+
+@pytest.mark.parametrize("name", ["channel16", "walls13x10x7"])
+def test_setup_again_matches_fresh_setup(name):
+    """gmres_pfmg on one device handle set up again for a sequence of
+    implicit operators (viscosities and velocity fields change, so the
+    symmetric part's strengths move): each setup's hierarchy and solve are
+    those of a fresh handle, bit for bit, and meet the direct solution."""
+    print(f"\n[gmres setup again {name}]", flush=True)
+    cfg = CASES[name]()
+    to_dev = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to(DEV)
+    seq = [inputs.implicit_problem(cfg, seed=s, visc=v) for s, v in ((1, 0.05), (2, 0.002), (3, 0.5), (4, 0.05))]
+    ds = DeviceSolve(cfg, method=GMRES_PFMG)
+    for q, P in enumerate(seq):
+        d_opr, d_rhs = to_dev(P.opr), to_dev(P.rhs)
+        ds.setup(d_opr, maxit=200, rcvg=1e-10)
+        fresh = DeviceSolve(cfg, method=GMRES_PFMG)
+        fresh.setup(d_opr, maxit=200, rcvg=1e-10)
+        ha, hb = ds.hooks(), fresh.hooks()
+        assert ha.nlevels == hb.nlevels
+        assert [ha.level_info(l) for l in range(ha.nlevels)] == [hb.level_info(l) for l in range(hb.nlevels)]
+        for l in range(ha.nlevels):
+            ca, cb = ha.level_coefs(l), hb.level_coefs(l)
+            for k in ca:
+                assert np.array_equal(ca[k], cb[k]), (q, l, k)
+        xa, xb = torch.zeros_like(d_rhs), torch.zeros_like(d_rhs)
+        ita, _ = ds.solve(d_rhs, xa)
+        itb, _ = fresh.solve(d_rhs, xb)
+        assert ita == itb and torch.equal(xa, xb), q
+        st = ds.stats()
+        x = interior(xa.cpu().numpy().reshape(P.rhs.shape, order="F"), cfg)
+        xr = _direct(cfg, P)
+        print(f"  setup {q}: it {ita} true {st['true_rerr']:.2e}", flush=True)
+        assert st["true_rerr"] <= 1e-10 and np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr)
+        fresh.destroy()
+    ds.destroy()
