@@ -77,11 +77,12 @@ __global__ void __launch_bounds__(NT) k_level0(Geo g, Ghost gh, Slots sl, const 
     __shared__ int sg[NT];
     double asym = 0.0, bad = 0.0;
     const int n3[3] = {g.n0, g.n1, g.n2}, per3[3] = {g.p0, g.p1, g.p2};
-    const int N = (int)g.N;  // cell indices fit in int (setup rejects 2^31 cells)
-    // the loop is uniform over the block (barriers inside)
-    for (int pb = blockIdx.x * NT; pb < N; pb += gridDim.x * NT) {
-        const int p = pb + threadIdx.x;
-        const bool ok = p < N;
+    // cell indices fit in int (setup rejects 2^31 cells); the loop counter is
+    // 64-bit so that its last step past N cannot overflow. The loop is uniform
+    // over the block (barriers inside).
+    for (long long pb = (long long)blockIdx.x * NT; pb < g.N; pb += (long long)gridDim.x * NT) {
+        const bool ok = pb + threadIdx.x < g.N;
+        const int p = ok ? (int)(pb + threadIdx.x) : 0;
         int c3[3] = {0, 0, 0};
         if (ok) {
             const unsigned rq = (unsigned)p / (unsigned)g.n0;

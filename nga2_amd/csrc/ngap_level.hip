@@ -227,7 +227,9 @@ __global__ void __launch_bounds__(NT) k_aniso_count(Geo g, const double *__restr
     const int n[3] = {g.n0, g.n1, g.n2}, per[3] = {g.p0, g.p1, g.p2};
     double cnt = 0.0, an = 0.0;
     const int stride[3] = {1, g.n0, g.n0 * g.n1};
-    for (int p = blockIdx.x * NT + threadIdx.x; p < (int)g.N; p += gridDim.x * NT) {
+    // 64-bit loop counter (its last step past N must not overflow); p fits in int
+    for (long long pl = (long long)blockIdx.x * NT + threadIdx.x; pl < g.N; pl += (long long)gridDim.x * NT) {
+        const int p = (int)pl;
         const unsigned r = (unsigned)p / (unsigned)g.n0;
         const int i = p - (int)r * g.n0, j = (int)(r % (unsigned)g.n1), k = (int)(r / (unsigned)g.n1);
         const int c3[3] = {i, j, k};
