@@ -382,11 +382,25 @@ struct alignas(8) F2 {
     float a, b;
 };
 
-// x += alpha p ; r -= alpha q ; partial ||r||^2 ; optional FP32 copy of r
+template <typename Tz>
+struct Pair;
+template <>
+struct Pair<double> {
+    typedef D2 type;
+};
+template <>
+struct Pair<float> {
+    typedef F2 type;
+};
+
+// x += alpha p ; r -= alpha q ; partial ||r||^2 ; optional FP32 copy of r.
+// Tp: the type p is stored in (FP32 in the mixed solve)
+template <typename Tp>
 __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restrict__ x, double *__restrict__ r,
-                                                  const double *__restrict__ p, const double *__restrict__ q,
+                                                  const Tp *__restrict__ p, const double *__restrict__ q,
                                                   const double *__restrict__ scal, float *__restrict__ r32,
                                                   double *__restrict__ part) {
+    typedef typename Pair<Tp>::type PP;
     __shared__ double sm[NT / 64];
     const double a = scal[S_ALPHA];
     double acc = 0.0;
@@ -395,9 +409,10 @@ __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restric
         D2 rv = reinterpret_cast<const D2 *>(r)[i];
         if (a != 0.0) {
             D2 xv = reinterpret_cast<const D2 *>(x)[i];
-            const D2 pv = reinterpret_cast<const D2 *>(p)[i], qv = reinterpret_cast<const D2 *>(q)[i];
-            xv.a += a * pv.a;
-            xv.b += a * pv.b;
+            const PP pv = reinterpret_cast<const PP *>(p)[i];
+            const D2 qv = reinterpret_cast<const D2 *>(q)[i];
+            xv.a += a * (double)pv.a;
+            xv.b += a * (double)pv.b;
             rv.a -= a * qv.a;
             rv.b -= a * qv.b;
             reinterpret_cast<D2 *>(x)[i] = xv;
@@ -411,7 +426,7 @@ __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restric
         const long long i = n - 1;
         double rv = r[i];
         if (a != 0.0) {
-            x[i] += a * p[i];
+            x[i] += a * (double)p[i];
             rv -= a * q[i];
             r[i] = rv;
         }
@@ -421,17 +436,6 @@ __global__ void __launch_bounds__(NT) k_update_xr(long long n, double *__restric
     double t = block_sum(acc, sm);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
-
-template <typename Tz>
-struct Pair;
-template <>
-struct Pair<double> {
-    typedef D2 type;
-};
-template <>
-struct Pair<float> {
-    typedef F2 type;
-};
 
 // <r,z> and (optionally) <z,q>
 template <typename Tz>
@@ -468,29 +472,33 @@ __global__ void __launch_bounds__(NT) k_dot_rz(long long n, const double *__rest
     }
 }
 
-template <typename Tz>
-__global__ void __launch_bounds__(NT) k_update_p(long long n, double *__restrict__ p, const Tz *__restrict__ z,
+// p = z + beta p (copy: p = z), formed in FP64 and stored as Tp: FP32 in the
+// mixed solve, whose directions come from the FP32 cycle (the x and r updates
+// use the stored p exactly, so they stay consistent with each other)
+template <typename Tz, typename Tp>
+__global__ void __launch_bounds__(NT) k_update_p(long long n, Tp *__restrict__ p, const Tz *__restrict__ z,
                                                  const double *__restrict__ scal, int copy) {
     typedef typename Pair<Tz>::type P;
+    typedef typename Pair<Tp>::type PP;
     const double be = copy ? 0.0 : scal[S_BETA];
     const long long n2 = n >> 1;
     for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n2; i += (long long)gridDim.x * NT) {
         const P zv = reinterpret_cast<const P *>(z)[i];
-        D2 pv;
+        double a, b;
         if (copy) {
-            pv.a = (double)zv.a;
-            pv.b = (double)zv.b;
+            a = (double)zv.a;
+            b = (double)zv.b;
         } else {
-            pv = reinterpret_cast<const D2 *>(p)[i];
-            pv.a = (double)zv.a + be * pv.a;
-            pv.b = (double)zv.b + be * pv.b;
+            const PP pv = reinterpret_cast<const PP *>(p)[i];
+            a = (double)zv.a + be * (double)pv.a;
+            b = (double)zv.b + be * (double)pv.b;
         }
-        reinterpret_cast<D2 *>(p)[i] = pv;
+        reinterpret_cast<PP *>(p)[i] = PP{(Tp)a, (Tp)b};
     }
     if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
         const long long i = n - 1;
         const double zv = (double)z[i];
-        p[i] = copy ? zv : zv + be * p[i];
+        p[i] = (Tp)(copy ? zv : zv + be * (double)p[i]);
     }
 }
 
@@ -675,6 +683,7 @@ struct ngap_t {
     int fail_alloc = 0;      // > 0: the fail_alloc-th next device allocation fails (ngapt_fail_alloc)
     Staging stg;             // pinned staging of ngap_solve, made on its first call
     int relax_keep = 1;      // levels 0..relax_keep-1 are always smoothed (set_relax)
+    bool p32 = false;        // p holds FP32 values (the mixed solve's directions), set at each restart
     long long zero_chunks = 0;  // guess chunks of the last ngap_solve found all zero (not copied)
     // the last setup completed with these options; its levels and coarsening
     // directions are then the first guess of the next setup's hierarchy
@@ -1537,6 +1546,31 @@ int precond_stage(ngap_t *h, bool mixed, bool stage, bool *z64) {
 }
 
 // r, z -> rho (restart: p = z)
+// p = z + beta p (copy: p = z) with z in FP32 or FP64 and p in the solve's
+// direction precision (h->p32)
+void launch_update_p(ngap_t *h, bool z64, int copy) {
+    const long long N = h->g0.N;
+    const int nb = grid_for(N);
+    hipStream_t s = h->stream;
+    const double *sc = h->scal;
+    float *p32 = reinterpret_cast<float *>(h->p);
+    if (h->p32) {
+        if (z64)
+            hipLaunchKernelGGL((k_update_p<double, float>), dim3(nb), dim3(NT), 0, s, N, p32, (const double *)h->z,
+                               sc, copy);
+        else
+            hipLaunchKernelGGL((k_update_p<float, float>), dim3(nb), dim3(NT), 0, s, N, p32, (const float *)h->z32,
+                               sc, copy);
+    } else {
+        if (z64)
+            hipLaunchKernelGGL((k_update_p<double, double>), dim3(nb), dim3(NT), 0, s, N, h->p, (const double *)h->z,
+                               sc, copy);
+        else
+            hipLaunchKernelGGL((k_update_p<float, double>), dim3(nb), dim3(NT), 0, s, N, h->p,
+                               (const float *)h->z32, sc, copy);
+    }
+}
+
 int restart_dir(ngap_t *h, bool mixed, bool first, bool stage) {
     long long N = h->g0.N;
     int nb = grid_for(N);
@@ -1551,12 +1585,10 @@ int restart_dir(ngap_t *h, bool mixed, bool first, bool stage) {
                            (const double *)h->z, (const double *)nullptr, h->parts, (double *)nullptr);
     hipLaunchKernelGGL(k_fin_rho, dim3(1), dim3(NT), 0, h->stream, (const double *)h->parts, nb, h->scal, h->flags,
                        first ? 1 : 0);
-    if (!z64)
-        hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, h->stream, N, h->p, (const float *)h->z32,
-                           (const double *)h->scal, 1);
-    else
-        hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, h->stream, N, h->p, (const double *)h->z,
-                           (const double *)h->scal, 1);
+    // the mixed solve keeps its directions in FP32 (in the p buffer), the
+    // FP64 solve (and a mixed solve fallen back to FP64) in FP64
+    h->p32 = mixed;
+    launch_update_p(h, z64, 1);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1610,12 +1642,7 @@ int next_dir(ngap_t *h, bool mixed, bool stage, bool flexible) {
                            h->parts + MAX_PARTS);
     hipLaunchKernelGGL(k_fin_beta, dim3(1), dim3(NT), 0, s, (const double *)h->parts,
                        (const double *)(h->parts + MAX_PARTS), nb, h->scal, h->flags, flexible ? 1 : 0);
-    if (!z64)
-        hipLaunchKernelGGL(k_update_p<float>, dim3(nb), dim3(NT), 0, s, N, h->p, (const float *)h->z32,
-                           (const double *)h->scal, 0);
-    else
-        hipLaunchKernelGGL(k_update_p<double>, dim3(nb), dim3(NT), 0, s, N, h->p, (const double *)h->z,
-                           (const double *)h->scal, 0);
+    launch_update_p(h, z64, 0);
     HIPCHK(hipGetLastError());
     return 0;
 }
@@ -1689,12 +1716,22 @@ int run_pcg(ngap_t *h) {
             restart = false;
         }
         Level &L0 = h->lev[0];
-        launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->p, h->q, h->parts);
-        hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, h->apply_parts, h->scal,
-                           h->flags);
-        hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
-                           (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
-                           h->parts + MAX_PARTS);
+        if (h->p32) {
+            const float *p32 = reinterpret_cast<const float *>(h->p);
+            launch_apply_x32(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, p32, h->q, h->parts);
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, h->apply_parts,
+                               h->scal, h->flags);
+            hipLaunchKernelGGL(k_update_xr<float>, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, p32,
+                               (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
+                               h->parts + MAX_PARTS);
+        } else {
+            launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->p, h->q, h->parts);
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, h->apply_parts,
+                               h->scal, h->flags);
+            hipLaunchKernelGGL(k_update_xr<double>, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
+                               (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
+                               h->parts + MAX_PARTS);
+        }
         hipLaunchKernelGGL(k_fin_sum, dim3(1), dim3(NT), 0, s, (const double *)(h->parts + MAX_PARTS), nb, h->scal,
                            (int)S_RR);
         HIPCHK(hipGetLastError());
@@ -2654,8 +2691,9 @@ int ngapt_time_kernel(ngap_t *h, int kid, int prec, int reps, double *avg_ms, do
             }
             case NGAPT_K_UPDATE: {
                 int nb = grid_for(N);
-                hipLaunchKernelGGL(k_update_xr, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
-                                   (const double *)h->q, (const double *)h->scal, (float *)nullptr, h->parts);
+                hipLaunchKernelGGL(k_update_xr<double>, dim3(nb), dim3(NT), 0, s, N, h->x, h->r,
+                                   (const double *)h->p, (const double *)h->q, (const double *)h->scal,
+                                   (float *)nullptr, h->parts);
                 return 0;
             }
             case NGAPT_K_EXACT:

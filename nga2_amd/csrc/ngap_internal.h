@@ -15,6 +15,10 @@ int grid_parts(long long n);
 template <typename T>
 void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *x, T *y,
                   double *part);
+// y = A x with the FP64 operator and x stored in FP32 (the mixed solve's
+// search directions); part as in launch_apply
+void launch_apply_x32(hipStream_t s, const Geo &g, const double *c, const double *a0, const double *a1,
+                      const double *a2, const float *x, double *y, double *part);
 // one symmetric-order Gauss-Seidel sweep, out of place (vin == nullptr: zero
 // initial iterate); c1 = colour updated first (0 red, 1 black)
 template <typename T>

@@ -6,6 +6,8 @@
 // Blocks tile the x-y plane; the SpMV and the sweep march through a chunk of
 // z planes so each array element is read once and z neighbours come from
 // registers.
+#include <type_traits>
+
 #include "ngap_internal.h"
 
 namespace ngap {
@@ -47,10 +49,12 @@ __device__ __forceinline__ void tile_of(int gx, int gy, int &bx, int &by, int &b
 // ------------------------------------------------------------------ SpMV
 // y = A x with the plane march; part (optional) gets one FP64 partial of
 // <x, y> per block. Block (TX, TY) tiles x-y; blockIdx.z selects a z chunk.
-template <typename T>
+// TX: the type x is stored in (FP32 search directions of the mixed solve
+// with the FP64 operator); the product is formed in T.
+template <typename T, typename TX = T>
 __global__ void __launch_bounds__(256) k_apply_t(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                  const T *__restrict__ a1, const T *__restrict__ a2,
-                                                 const T *__restrict__ x, T *__restrict__ y,
+                                                 const TX *__restrict__ x, T *__restrict__ y,
                                                  double *__restrict__ part, int kc, int gx, int gy) {
     __shared__ double sm[4];
     int bx, by, bz;
@@ -70,19 +74,19 @@ __global__ void __launch_bounds__(256) k_apply_t(Geo g, const T *__restrict__ c,
         const int ym = (jm - j) * n0, yp = (jp - j) * n0;  // y offsets
         int km = nbm(k0, n2, g.p2);
         const int col = row + i;
-        T xm = km >= 0 ? x[km * plane + col] : T(0);
-        T xc = x[k0 * plane + col];
+        T xm = km >= 0 ? T(x[km * plane + col]) : T(0);
+        T xc = T(x[k0 * plane + col]);
         T a2c = a2[k0 * plane + col];
         for (int k = k0; k < k1; k++) {
             const int p = k * plane + col;
             const int kp = nbp(k, n2, g.p2);
-            const T xp = kp >= 0 ? x[kp * plane + col] : T(0);
+            const T xp = kp >= 0 ? T(x[kp * plane + col]) : T(0);
             const T a2p = kp >= 0 ? a2[kp * plane + col] : T(0);
             T s = c[p] * xc;
-            if (im >= 0) s += a0[p] * x[p + om];
-            if (ip >= 0) s += a0[p + op] * x[p + op];
-            if (jm >= 0) s += a1[p] * x[p + ym];
-            if (jp >= 0) s += a1[p + yp] * x[p + yp];
+            if (im >= 0) s += a0[p] * T(x[p + om]);
+            if (ip >= 0) s += a0[p + op] * T(x[p + op]);
+            if (jm >= 0) s += a1[p] * T(x[p + ym]);
+            if (jp >= 0) s += a1[p + yp] * T(x[p + yp]);
             if (km >= 0) s += a2c * xm;
             if (kp >= 0) s += a2p * xp;
             y[p] = s;
@@ -141,10 +145,12 @@ __device__ __forceinline__ Vec<T, CPL> vzero() {
 // Sums are accumulated in the same order as the tiled kernels.
 enum { RW_APPLY = 0, RW_RESID = 1, RW_SMOOTH0 = 2 };
 
-template <typename T, int CPL, int MODE>
+// TX: the type x is stored in (RW_APPLY of the mixed solve: FP32 search
+// directions, FP64 operator and product).
+template <typename T, int CPL, int MODE, typename TX = T>
 __global__ void __launch_bounds__(256) k_stencil_rw(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
                                                     const T *__restrict__ a1, const T *__restrict__ a2,
-                                                    const T *__restrict__ f, const T *__restrict__ x,
+                                                    const T *__restrict__ f, const TX *__restrict__ x,
                                                     T *__restrict__ y, double *__restrict__ part, int c1, int kc,
                                                     int gx, int gy) {
     __shared__ double sm[4];
@@ -167,11 +173,19 @@ __global__ void __launch_bounds__(256) k_stencil_rw(Geo g, const T *__restrict__
 #pragma unroll
             for (int e = 0; e < CPL; e++) u.v[e] = fq.v[e] / cq.v[e];
             return u;
-        } else {
+        } else if constexpr (std::is_same<TX, T>::value) {
             return vld<T, CPL>(x + q);
+        } else {
+            // CPL narrower values in one load of CPL * sizeof(TX) bytes
+            typedef TX xv_t __attribute__((ext_vector_type(CPL)));
+            const xv_t t = *reinterpret_cast<const xv_t *>(x + q);
+            V r;
+#pragma unroll
+            for (int e = 0; e < CPL; e++) r.v[e] = T(t[e]);
+            return r;
         }
     };
-    auto opnd1 = [&](int q) -> T { return MODE == RW_SMOOTH0 ? f[q] / c[q] : x[q]; };
+    auto opnd1 = [&](int q) -> T { return MODE == RW_SMOOTH0 ? f[q] / c[q] : T(x[q]); };
     if (i0 < n0 && j < n1 && k0 < k1) {
         const int jm = nbm(j, n1, g.p1), jp = nbp(j, n1, g.p1);
         const int il = nbm(i0, n0, g.p0), ir = nbp(i0 + CPL - 1, n0, g.p0);
@@ -1305,6 +1319,20 @@ void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
     tile_dims(g, tx, ty);
     hipLaunchKernelGGL(k_apply_t<T>, dim3(a.blocks), dim3(tx, ty), 0, s, g, c, a0, a1, a2, x, y, part, a.kc, a.gx,
                        a.gy);
+}
+
+void launch_apply_x32(hipStream_t s, const Geo &g, const double *c, const double *a0, const double *a1,
+                      const double *a2, const float *x, double *y, double *part) {
+    const ApplyCfg a = apply_cfg<double>(g);
+    if (a.rw) {
+        hipLaunchKernelGGL((k_stencil_rw<double, 2, RW_APPLY, float>), dim3(a.blocks), dim3(256), 0, s, g, c, a0, a1,
+                           a2, (const double *)nullptr, x, y, part, 0, a.kc, a.gx, a.gy);
+        return;
+    }
+    int tx, ty;
+    tile_dims(g, tx, ty);
+    hipLaunchKernelGGL((k_apply_t<double, float>), dim3(a.blocks), dim3(tx, ty), 0, s, g, c, a0, a1, a2, x, y, part,
+                       a.kc, a.gx, a.gy);
 }
 
 template <typename T>
