@@ -556,12 +556,21 @@ __global__ void __launch_bounds__(NT) k_fin_max(const double *__restrict__ parts
     if (threadIdx.x == 0) scal[slot] = s;
 }
 
-// alpha = rho / <p,Ap>; 0 and a flag on breakdown
-__global__ void __launch_bounds__(NT) k_fin_alpha(const double *__restrict__ parts, int n, double *__restrict__ scal,
-                                                  int *__restrict__ flags) {
-    __shared__ double sm[NT / 64];
-    double pq = sum_parts(parts, n, sm);
+// alpha = rho / <p,Ap>; 0 and a flag on breakdown. The SpMV leaves up to
+// MAX_PARTS partials, so this finalize runs 1024 threads (8 loads each at
+// 256^3 instead of 32), in a fixed order.
+constexpr int NT_FIN = 1024;
+__global__ void __launch_bounds__(NT_FIN) k_fin_alpha(const double *__restrict__ parts, int n,
+                                                      double *__restrict__ scal, int *__restrict__ flags) {
+    __shared__ double sm[NT_FIN / 64];
+    double v = 0.0;
+    for (int q = threadIdx.x; q < n; q += NT_FIN) v += parts[q];
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+    __syncthreads();
     if (threadIdx.x == 0) {
+        double pq = 0.0;
+        for (int w = 0; w < NT_FIN / 64; w++) pq += sm[w];
         double rho = scal[S_RHO];
         double a = rho / pq;
         if (!(pq > 0.0) || !(rho > 0.0) || !isfinite(a)) {
@@ -1719,14 +1728,14 @@ int run_pcg(ngap_t *h) {
         if (h->p32) {
             const float *p32 = reinterpret_cast<const float *>(h->p);
             launch_apply_x32(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, p32, h->q, h->parts);
-            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, h->apply_parts,
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT_FIN), 0, s, (const double *)h->parts, h->apply_parts,
                                h->scal, h->flags);
             hipLaunchKernelGGL(k_update_xr<float>, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, p32,
                                (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
                                h->parts + MAX_PARTS);
         } else {
             launch_apply<double>(s, L0.g, L0.d.c, L0.d.a0, L0.d.a1, L0.d.a2, h->p, h->q, h->parts);
-            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT), 0, s, (const double *)h->parts, h->apply_parts,
+            hipLaunchKernelGGL(k_fin_alpha, dim3(1), dim3(NT_FIN), 0, s, (const double *)h->parts, h->apply_parts,
                                h->scal, h->flags);
             hipLaunchKernelGGL(k_update_xr<double>, dim3(nb), dim3(NT), 0, s, N, h->x, h->r, (const double *)h->p,
                                (const double *)h->q, (const double *)h->scal, mixed ? h->r32 : (float *)nullptr,
