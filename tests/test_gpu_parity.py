@@ -19,8 +19,9 @@ from oracle.orc import Oracle, interior
 pytestmark = pytest.mark.gpu
 
 
-def _box(nx, ny, nz, per, walls=True):
-    cfg = Config(uniform(nx), uniform(ny), uniform(nz), no=1, xper=per[0], yper=per[1], zper=per[2])
+def _box(nx, ny, nz, per, walls=True, L=(1.0, 1.0, 1.0)):
+    cfg = Config(uniform(nx, L[0]), uniform(ny, L[1]), uniform(nz, L[2]), no=1, xper=per[0], yper=per[1],
+                 zper=per[2])
     if walls:
         cfg.VF[:] = 0.0
         cfg.VF[cfg.interior] = 1.0
@@ -68,6 +69,13 @@ CASES = {
     # (one x-row per wavefront), with walls in y, and in x (row ends)
     "rows256x16x256": lambda: _box(256, 16, 256, (True, False, True)),
     "rows128x64x128": lambda: _box(128, 64, 128, (False, True, True)),
+    # rows that end inside a wavefront (260 and 200 cells: the last wave of
+    # a row holds 1 or 50 active lanes in FP32), walled and periodic in x;
+    # cells of equal size in all directions
+    "rows260x12x20": lambda: _box(260, 12, 20, (False, False, False), L=(1.0, 12 / 260, 20 / 260)),
+    "rows200x10x16": lambda: _box(200, 10, 16, (True, False, True), L=(1.0, 10 / 200, 16 / 200)),
+    # a line of cells along z (x and y one and three cells wide)
+    "line1x3x300": lambda: _box(1, 3, 300, (True, False, False)),
     # identity rows (opr(1) == 0): a solid block inside a box with walls in
     # y, and solid spheres immersed in a periodic cube
     "masked_block16x11x8": lambda: _masked_box(16, 11, 8, (True, False, True), ((5, 9), (3, 7), (2, 5))),
