@@ -39,7 +39,10 @@ int ngapt_apply(ngap_t *h, int l, int prec, const double *x, double *y);
 /* one symmetric Gauss-Seidel sweep; order bit 0: colour updated first
  * (0 red, 1 black); bit 1: start from a zero iterate (v ignored on input);
  * bit 2: two sweeps of that order in one launch (the post-smoothing pair);
- * bits 1 and 2: the first two sweeps from a zero iterate in one launch */
+ * bits 1 and 2: the first two sweeps from a zero iterate in one launch;
+ * without bits 1 and 2, bit 3: the 64-wide tiled sweep kernel also on levels
+ * whose rows fit one workgroup row, and bit 4: the sweep kernel itself from a
+ * zero iterate */
 int ngapt_rbgs(ngap_t *h, int l, int prec, const double *f, double *v, int order);
 /* residual + restriction, prolongation + correction of level l; prec bit 0:
  * FP32, bit 1: the one-cell-per-thread transfer kernels on every level (the

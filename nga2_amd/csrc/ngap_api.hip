@@ -2269,8 +2269,9 @@ static int t_rbgs(ngap_t *h, int l, const double *f, double *v, int order) {
         launch_smooth0<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dout, order & 1);
     else if (order & 4)  // two sweeps of the same order in one launch (the post-smoothing pair)
         launch_sweep2<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dout, order & 1);
-    else
-        launch_sweep<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, dv, dout, order & 1);
+    else  // bit 3: the 64-wide tiles on every level; bit 4: the single sweep from a zero iterate
+        launch_sweep<T>(h->stream, L.g, D.c, D.a0, D.a1, D.a2, df, (order & 16) ? (const T *)nullptr : dv, dout,
+                        order & 1, (order & 8) != 0);
     HIPCHK(hipGetLastError());
     return down<T>(h->stream, tmp, dout, n, v);
 }

@@ -20,10 +20,11 @@ void launch_apply(hipStream_t s, const Geo &g, const T *c, const T *a0, const T 
 void launch_apply_x32(hipStream_t s, const Geo &g, const double *c, const double *a0, const double *a1,
                       const double *a2, const float *x, double *y, double *part);
 // one symmetric-order Gauss-Seidel sweep, out of place (vin == nullptr: zero
-// initial iterate); c1 = colour updated first (0 red, 1 black)
+// initial iterate); c1 = colour updated first (0 red, 1 black). Levels of rows
+// <= 128 cells run whole-row tiles (k_sweep_row) unless tiles is set.
 template <typename T>
 void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
-                  const T *vin, T *vout, int c1);
+                  const T *vin, T *vout, int c1, bool tiles = false);
 // two fused sweeps of the same colour order (vin != vout)
 template <typename T>
 void launch_sweep2(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,

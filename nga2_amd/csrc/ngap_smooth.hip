@@ -647,6 +647,145 @@ __global__ void __launch_bounds__(SX *SY) k_sweep(Geo g, const T *__restrict__ c
     }
 }
 
+// ------------------------------------------------------------------ sweep over whole rows
+// The sweep of k_sweep on levels whose x rows fit one workgroup row
+// (n0 <= NX, NX = 32, 64 or 128): a workgroup of NX x NY threads (NX NY =
+// 1024) holds whole rows, so every x neighbour is in the tile (across the row
+// end for periodic x) and only y has a recompute ring (NY - 2 output rows).
+// On 128-cell rows k_sweep's 62-column tiles need three tile columns per row
+// and half of the threads do no output; here a quarter do none. Per cell the
+// arithmetic of k_sweep in the same order; a neighbour beyond a wall
+// contributes 0 * 0 there and here, so the two kernels are bit-identical.
+template <typename T, bool ZERO_IN, int NX>
+__global__ void __launch_bounds__(1024) k_sweep_row(Geo g, const T *__restrict__ c, const T *__restrict__ a0,
+                                                    const T *__restrict__ a1, const T *__restrict__ a2,
+                                                    const T *__restrict__ f, const T *__restrict__ vin,
+                                                    T *__restrict__ vout, int c1, int kc, int gy) {
+    constexpr int NY = 1024 / NX;
+    __shared__ T s_vin[3][NY][NX];
+    __shared__ T s_a0[3][NY][NX];
+    __shared__ T s_a1[3][NY][NX];
+    __shared__ T s_v1[2][NY][NX];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+    const int plane = n0 * n1;
+    int bx, by, bz;
+    tile_of(1, gy, bx, by, bz);
+    const int ju = by * (NY - 2) - 1 + ty;
+    const int jj = wrapc(ju, n1, g.p1);
+    const bool inrow = tx < n0;  // lanes past the row end hold zeros
+    const bool valid = inrow && jj >= 0;
+    const int col = valid ? jj * n0 + tx : 0;
+    // in-row neighbours in the tile (-1: beyond a wall, or a lane past the row)
+    const int txm = inrow ? nbm(tx, n0, g.p0) : -1;
+    const int txp = inrow ? nbp(tx, n0, g.p0) : -1;
+    const bool bot = ty == 0, top = ty == NY - 1;
+    const bool core = !bot && !top && ju < n1 && inrow;
+    // the one cell outside the tile a ring row couples to (-1: none)
+    int cout = -1;
+    if (valid) {
+        if (bot) { int q = wrapc(jj - 1, n1, g.p1); cout = q >= 0 ? q * n0 + tx : -1; }
+        if (top) { int q = wrapc(jj + 1, n1, g.p1); cout = q >= 0 ? q * n0 + tx : -1; }
+    }
+    const int par = (tx + jj) & 1;  // colour of the cell on plane k is (par + k) & 1
+    const int k0 = bz * kc;
+    const int k1 = min(n2, k0 + kc);
+
+    auto kwrap = [&](int k) { return wrapc(k, n2, g.p2); };
+    auto ld = [&](const T *__restrict__ a, int kk) -> T { return (valid && kk >= 0) ? a[kk * plane + col] : T(0); };
+    auto ldo = [&](const T *__restrict__ a, int kk) -> T { return (cout >= 0 && kk >= 0) ? a[kk * plane + cout] : T(0); };
+    // an x neighbour's entry of an LDS plane (0 beyond a wall)
+    auto lx = [&](const T (*S)[NX], int q) -> T { return q >= 0 ? S[ty][q] : T(0); };
+
+    int s = k0 - 2;
+    int kA = kwrap(s + 1);
+    T vin_s = ZERO_IN ? T(0) : ld(vin, kwrap(s));
+    T c_A = valid && kA >= 0 ? c[kA * plane + col] : T(1);
+    T f_A = ld(f, kA), a0_A = ld(a0, kA), a1_A = ld(a1, kA), a2_A = ld(a2, kA);
+    T vin_A = ZERO_IN ? T(0) : ld(vin, kA);
+    T vo_A = ZERO_IN ? T(0) : ldo(vin, kA);
+    T ao_A = top ? ldo(a1, kA) : T(0);
+    int kB = kwrap(s + 2);
+    T vin_B = ZERO_IN ? T(0) : ld(vin, kB);
+    T a2_B = ld(a2, kB);
+    T c_s = T(1), f_s = T(0), a0_s = T(0), a1_s = T(0), a2_s = T(0);
+    T v1_m = T(0), v1_0 = T(0);
+    int slot3 = 0;
+    for (; s < k1; s++) {
+        const int kN = kwrap(s + 2), kN2 = kwrap(s + 3);
+        const T c_N = valid && kN >= 0 ? c[kN * plane + col] : T(1);
+        const T f_N = ld(f, kN), a0_N = ld(a0, kN), a1_N = ld(a1, kN);
+        const T vo_N = ZERO_IN ? T(0) : ldo(vin, kN);
+        const T ao_N = top ? ldo(a1, kN) : T(0);
+        const T vin_N2 = ZERO_IN ? T(0) : ld(vin, kN2);
+        const T a2_N2 = ld(a2, kN2);
+        if (!ZERO_IN) s_vin[slot3][ty][tx] = vin_A;
+        s_a0[slot3][ty][tx] = a0_A;
+        s_a1[slot3][ty][tx] = a1_A;
+        lds_barrier();
+        const int ks = kwrap(s);
+        // first colour on plane s+1 (ring rows included)
+        T v1_p = T(0);
+        if (valid && kA >= 0 && ((par + kA) & 1) == c1) {
+            T sum = f_A;
+            if (!ZERO_IN) {
+                const T xm = lx(s_vin[slot3], txm);
+                const T xp = lx(s_vin[slot3], txp);
+                const T ym = bot ? vo_A : s_vin[slot3][ty - 1][tx];
+                const T yp = top ? vo_A : s_vin[slot3][ty + 1][tx];
+                const T a0p = lx(s_a0[slot3], txp);
+                const T a1p = top ? ao_A : s_a1[slot3][ty + 1][tx];
+                sum -= a0_A * xm;
+                sum -= a0p * xp;
+                sum -= a1_A * ym;
+                sum -= a1p * yp;
+                sum -= a2_A * vin_s;
+                sum -= a2_B * vin_B;
+            }
+            v1_p = sum / c_A;
+        }
+        s_v1[(s + 1) & 1][ty][tx] = v1_p;
+        // second colour on plane s (core rows only)
+        if (s >= k0 && core && ks >= 0) {
+            T out;
+            if (((par + ks) & 1) == c1) {
+                out = v1_0;
+            } else {
+                const int sl = (slot3 + 2) % 3;
+                T sum = f_s;
+                sum -= a0_s * lx(s_v1[s & 1], txm);
+                sum -= lx(s_a0[sl], txp) * lx(s_v1[s & 1], txp);
+                sum -= a1_s * s_v1[s & 1][ty - 1][tx];
+                sum -= s_a1[sl][ty + 1][tx] * s_v1[s & 1][ty + 1][tx];
+                sum -= a2_s * v1_m;
+                sum -= a2_A * v1_p;
+                out = sum / c_s;
+            }
+            vout[ks * plane + col] = out;
+        }
+        vin_s = vin_A;
+        c_s = c_A;
+        f_s = f_A;
+        a0_s = a0_A;
+        a1_s = a1_A;
+        a2_s = a2_A;
+        kA = kN;
+        c_A = c_N;
+        f_A = f_N;
+        a0_A = a0_N;
+        a1_A = a1_N;
+        a2_A = a2_B;
+        vin_A = vin_B;
+        vo_A = vo_N;
+        ao_A = ao_N;
+        vin_B = vin_N2;
+        a2_B = a2_N2;
+        v1_m = v1_0;
+        v1_0 = v1_p;
+        slot3 = (slot3 + 1) % 3;
+    }
+}
+
 // ------------------------------------------------------------------ two fused sweeps
 // Two consecutive sweeps of the same colour order c1 (the post-smoothing of a
 // V(., 2) cycle) in one launch: four colour half-steps
@@ -1429,10 +1568,43 @@ static void launch_sweep_sy(hipStream_t s, const Geo &g, const T *c, const T *a0
 // 256^3 measured best among 64 x {4, 8, 16} tiles and 512..4096 groups. (A
 // row-wave variant with 16-byte loads and 14-row workgroups measured slower:
 // 183 against 137 us at 256^3, FP32.)
+// Whole-row tiles (k_sweep_row) on levels of rows <= 128 cells: about
+// `target` workgroups, z chunks of >= ROW_MIN_KC planes on large levels. With
+// fewer tiles per plane than k_sweep the same workgroup count leaves shorter
+// z chunks, hence fewer plane steps per workgroup.
+constexpr int ROW_MIN_KC = 8;
+
+template <typename T, int NX>
+static void launch_sweep_row(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2,
+                             const T *f, const T *vin, T *vout, int c1, long long target) {
+    constexpr int NY = 1024 / NX;
+    const int gy = (g.n1 + NY - 3) / (NY - 2);
+    long long gz = std::max<long long>(1, target / gy);
+    if (g.N >= (1 << 22) && gz > 1 && (g.n2 + gz - 1) / gz < ROW_MIN_KC)
+        gz = std::max<long long>(1, g.n2 / ROW_MIN_KC);
+    if (gz > g.n2) gz = g.n2;
+    int kc = (int)((g.n2 + gz - 1) / gz);
+    if (kc < 1) kc = 1;
+    const int nz = (g.n2 + kc - 1) / kc;
+    dim3 grid((unsigned)(gy * nz)), block(NX, NY);
+    if (vin)
+        hipLaunchKernelGGL((k_sweep_row<T, false, NX>), grid, block, 0, s, g, c, a0, a1, a2, f, vin, vout, c1, kc, gy);
+    else
+        hipLaunchKernelGGL((k_sweep_row<T, true, NX>), grid, block, 0, s, g, c, a0, a1, a2, f, (const T *)nullptr,
+                           vout, c1, kc, gy);
+}
+
 template <typename T>
 void launch_sweep(hipStream_t s, const Geo &g, const T *c, const T *a0, const T *a1, const T *a2, const T *f,
-                  const T *vin, T *vout, int c1) {
-    launch_sweep_sy<T, 16>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
+                  const T *vin, T *vout, int c1, bool tiles) {
+    if (!tiles && g.n0 <= 32)
+        launch_sweep_row<T, 32>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
+    else if (!tiles && g.n0 <= 64)
+        launch_sweep_row<T, 64>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
+    else if (!tiles && g.n0 <= 128)
+        launch_sweep_row<T, 128>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
+    else
+        launch_sweep_sy<T, 16>(s, g, c, a0, a1, a2, f, vin, vout, c1, 512);
 }
 
 // Two sweeps of colour order c1 in one launch (k_sweep2): 58 x 10 output
@@ -1504,7 +1676,7 @@ template void launch_bottom<float>(hipStream_t, const BotArgs<float> &);
     template void launch_apply<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
                                   T *, double *);                                                                  \
     template void launch_sweep<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *, const T *, \
-                                  const T *, T *, int);                                                            \
+                                  const T *, T *, int, bool);                                                      \
     template void launch_sweep2<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *,          \
                                    const T *, const T *, T *, int);                                                \
     template void launch_smooth0<T>(hipStream_t, const Geo &, const T *, const T *, const T *, const T *,          \

@@ -190,6 +190,26 @@ def test_fused_pair_from_zero(case, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_row_sweep_matches_tiled(case, prec):
+    """On levels whose rows fit one workgroup row (<= 128 cells) the sweep runs
+    whole-row tiles with a ring in y only; it equals the 64-wide tiled sweep
+    (ring in x and y) bit for bit, from an iterate and from zero, on every
+    level and in both colour orders."""
+    name, P, o, ps = case
+    hk = ps.hooks()
+    rng = np.random.default_rng(12)
+    for l in range(o.nlevels):
+        n = o.level_size(l)
+        x = rng.standard_normal(n)
+        f = rng.standard_normal(n)
+        for order in (0, 1):
+            for zero in (0, 16):
+                row = hk.rbgs(l, f, x, zero | order, prec=prec)
+                tiled = hk.rbgs(l, f, x, 8 | zero | order, prec=prec)
+                assert np.array_equal(row, tiled), (name, l, order, zero, np.abs(row - tiled).max())
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_vector_transfers_match_scalar(case, prec):
     """Restriction and prolongation with CPL cells per thread and 16-byte
     accesses (y/z coarsening, rows a multiple of CPL) equal the
