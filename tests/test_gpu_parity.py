@@ -210,6 +210,24 @@ def test_row_sweep_matches_tiled(case, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
+def test_zero_sweep_matches_stencil(case, prec):
+    """The first pre-smoothing sweep from a zero iterate runs as the sweep
+    kernel on levels of >= 2^20 cells with rows of <= 128 cells and as the
+    stencil form elsewhere; the two give the same bits on every level."""
+    name, P, o, ps = case
+    hk = ps.hooks()
+    rng = np.random.default_rng(13)
+    for l in range(o.nlevels):
+        n = o.level_size(l)
+        x = rng.standard_normal(n)
+        f = rng.standard_normal(n)
+        for order in (0, 1):
+            sweep = hk.rbgs(l, f, x, 16 | order, prec=prec)
+            stencil = hk.rbgs(l, f, x, 2 | order, prec=prec)
+            assert np.array_equal(sweep, stencil), (name, l, order, np.abs(sweep - stencil).max())
+
+
+@pytest.mark.parametrize("prec", [0, 1])
 def test_vector_transfers_match_scalar(case, prec):
     """Restriction and prolongation with CPL cells per thread and 16-byte
     accesses (y/z coarsening, rows a multiple of CPL) equal the
