@@ -756,8 +756,10 @@ def main():
     # post-smoothing pair k_sweep2<float,16,false>, the largest fine-level launch. "achieved"
     # counts the algorithmic bytes of its one pass over the arrays (7 x 4 B per cell);
     # "sweep_equiv_gbs" the bytes of the two single-sweep launches it replaces. The other
-    # fine-level kernels, the single sweep k_sweep<float> (levels 1-17 of the cycle) and the
-    # FP64 sweep and SpMV of the plain method are timed beside it.
+    # fine-level kernels, the single sweep k_sweep<float> (timed on the fine level; in the cycle
+    # it runs on the levels with rows of more than 128 cells below it, and the whole-row
+    # k_sweep_row on the others) and the FP64 sweep and SpMV of the plain method are timed
+    # beside it.
     hk = ds_mixed.hooks()
     rb_ms, rb_bytes = hk.time_kernel(_lib.K_RBGS, prec=1, reps=args.roofline_reps)
     r64_ms, r64_bytes = hk.time_kernel(_lib.K_RBGS, prec=0, reps=args.roofline_reps)
@@ -786,7 +788,7 @@ def main():
                                          "k_sweep2<float,16,true>: the fine level's pre-smoothing pair from a zero iterate",
                                          6.0 / 7.0 * rb_bytes + rb_bytes)
     roofline["sweep_fp32"] = entry(rb_ms, rb_bytes, "sweep_fp32_channel256.txt",
-                                   "k_sweep<float>: one fused red/black GS sweep (runs on levels 1-17 of the cycle)")
+                                   "k_sweep<float>: one fused red/black GS sweep (64 x 16 tiles), on the fine level")
     roofline["sweep_fp64"] = entry(r64_ms, r64_bytes, "sweep_fp64_channel256.txt",
                                    "k_sweep<double>: the plain method's sweep (RelaxType 2), fine level")
     roofline["spmv_fp64"] = entry(sp_ms, sp_bytes, "spmv_fp64_channel256.txt",
