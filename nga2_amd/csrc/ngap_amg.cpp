@@ -27,6 +27,7 @@
 #include <stdexcept>
 #include <system_error>
 #include <thread>
+#include <utility>
 
 namespace ngap {
 namespace amg {
@@ -316,6 +317,9 @@ void interp_rows(const Csr &A, const Graph &G, const std::vector<int> &cf, const
     std::vector<int> chat_pos(n, 0), sstamp(n, -1);
     std::vector<int> chat, cols;
     std::vector<double> wv, vals;
+    // entries of a strong F-neighbour's row that distribute to C-hat or to i,
+    // in row order: (C-hat position, or -1 for i; value)
+    std::vector<std::pair<int, double>> dk;
     TruncScratch tw;
     for (int i = lo; i < hi; i++) {
         const size_t before = pcol.size();
@@ -362,24 +366,32 @@ void interp_rows(const Csr &A, const Graph &G, const std::vector<int> &cf, const
                 if (k == i || chat_pos[k] || sstamp[k] != i) continue;
                 const double aik = A.val[q];
                 const double sgn = adiag[k] >= 0.0 ? 1.0 : -1.0;
+                // one pass over row k: the denominator and the entries it distributes to
                 double den = 0.0;
+                dk.clear();
                 for (int t = A.rp[k]; t < A.rp[k + 1]; t++) {
                     const int l = A.col[t];
-                    if ((chat_pos[l] || l == i) && A.val[t] * sgn < 0.0) den += A.val[t];
+                    const double v = A.val[t];
+                    if (v * sgn < 0.0) {
+                        if (chat_pos[l]) {
+                            dk.emplace_back(chat_pos[l] - 1, v);
+                            den += v;
+                        } else if (l == i) {
+                            dk.emplace_back(-1, v);
+                            den += v;
+                        }
+                    }
                 }
                 if (den == 0.0) {
                     diag += aik;
                     continue;
                 }
                 const double dist = aik / den;
-                for (int t = A.rp[k]; t < A.rp[k + 1]; t++) {
-                    const int l = A.col[t];
-                    if (A.val[t] * sgn < 0.0) {
-                        if (chat_pos[l])
-                            wv[chat_pos[l] - 1] += dist * A.val[t];
-                        else if (l == i)
-                            diag += dist * A.val[t];
-                    }
+                for (const auto &e : dk) {
+                    if (e.first >= 0)
+                        wv[e.first] += dist * e.second;
+                    else
+                        diag += dist * e.second;
                 }
             }
             cols.resize(chat.size());
@@ -404,8 +416,10 @@ Csr interp_ext_i(const Csr &A, const Graph &G, const std::vector<int> &cf, int p
     for (int i = 0; i < n; i++)
         if (cf[i] == C_PT) cmap[i] = nc++;
     std::vector<double> adiag(n);
-    for (int i = 0; i < n; i++) adiag[i] = diag_of(A, i);
     const int nt = nthreads_for(n, (long long)A.col.size());
+    par_rows(n, nt, [&](int lo, int hi, int) {
+        for (int i = lo; i < hi; i++) adiag[i] = diag_of(A, i);
+    });
     std::vector<std::vector<int>> pcol(nt), len(nt);
     std::vector<std::vector<double>> pval(nt);
     par_rows(n, nt, [&](int lo, int hi, int t) {
