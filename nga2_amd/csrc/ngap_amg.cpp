@@ -499,8 +499,8 @@ struct RowAcc {
     std::vector<double> hval, acc;
     RowAcc() : hkey(HCAP, -1), hval(HCAP) {}
 
-    // the distinct columns of row i of A B (ascending) in cols; with sums when
-    // vals is set (read back through column())
+    // the distinct columns of row i of A B in cols; with vals set, ascending
+    // and with their sums (read back through column())
     void row(const Csr &A, const Csr &B, int i, bool vals) {
         cols.clear();
         long long bound = 0;
@@ -545,7 +545,8 @@ struct RowAcc {
             // a later row reuses mark: clear this row's entries
             for (int j : cols) mark[j] = -1;
         }
-        std::sort(cols.begin(), cols.end());
+        // the counting pass needs the number of columns only
+        if (vals) std::sort(cols.begin(), cols.end());
     }
     double column(int j) const {
         if (dense) return acc[j];
